@@ -1,0 +1,263 @@
+// Breadth-first bucketed octree build — the device form of MortonTree.build
+// (mortonTree.js:107-162), one kernel per depth level, bit-identical node
+// order and contents.
+//
+// Level d holds nodes [start_d, start_d + count_d), each a segment [lo, hi)
+// of the Morton-sorted particles.  Per node (mortonTree.js:119-156):
+//   leaf   iff count <= bucket || d >= 10 || idx + 9 > maxNodes
+//          (numNodes after its own increment is idx + 1, so the pool rule is
+//          a function of the node index alone)
+//   else   children = the non-empty runs of the 3-bit digit at shift 27 - 3d,
+//          found by binary search (codes in a node share all higher digits,
+//          so the digit is monotone over [lo, hi)); 8 lanes search the 8
+//          octant boundaries of one node in parallel.
+// Children are numbered in BFS order = parent order x octant order, i.e. an
+// exclusive scan of child counts over the level; the scan runs in the same
+// kernel with a decoupled look-back over 256-node tiles taken in ticket
+// order.  A node's children are therefore contiguous: internal nodes store
+// (firstChild, nChild, octant mask) and the 64-byte layout is produced on
+// download.  The last tile of a level writes count_{d+1}.
+#include "gs_internal.h"
+
+namespace {
+
+constexpr uint64_t kAgg = 1ull << 32;
+constexpr uint64_t kPre = 2ull << 32;
+
+__device__ __forceinline__ uint64_t ld_agent64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ codes, uint32_t first,
+                                                uint32_t len, uint32_t target) {
+  while (len > 0) {
+    const uint32_t half = len >> 1;
+    if (codes[first + half] < target) {
+      first += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return first;
+}
+
+__global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
+    const uint32_t* __restrict__ codes, uint32_t n, DevTree t, DevScalars* sc, int d,
+    uint32_t bucket, uint64_t max_nodes, uint64_t* status, uint64_t* status_clear,
+    uint32_t clear_n) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_b[GS_BUILD_TILE][9];
+  __shared__ uint32_t s_wsum[GS_BUILD_TILE / 64];
+  __shared__ uint64_t s_prefix;
+
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t k = gtid; k < clear_n; k += gridDim.x * blockDim.x) status_clear[k] = 0;
+
+  const uint32_t count = sc->level_count[d];
+  const uint32_t start = sc->level_start[d];
+  if (count == 0) {
+    if (gtid == 0) {
+      sc->level_start[d + 1] = start;
+      sc->level_count[d + 1] = 0;
+    }
+    return;
+  }
+  const uint32_t ntiles = (count + GS_BUILD_TILE - 1) / GS_BUILD_TILE;
+  const uint32_t next_start = start + count;
+  const uint32_t size_d = __float_as_uint(sc->level_size[d]);
+  const uint32_t size_c = __float_as_uint(sc->level_size[d + 1]);
+  const int shift = 27 - 3 * d;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(&sc->tickets[4 + d], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= ntiles) break;
+
+    // phase 1: octant boundaries b_0..b_7 (+ hi), 8 lanes per node
+    for (int r = 0; r < GS_BUILD_TILE / 32; r++) {
+      const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
+      const uint32_t k = threadIdx.x & 7;
+      const uint32_t local = tile * GS_BUILD_TILE + j;
+      if (local < count) {
+        const uint32_t idx = start + local;
+        const int4 inf = t.info[idx];
+        const uint32_t lo = (uint32_t)inf.y, cnt = (uint32_t)inf.z;
+        const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
+        uint32_t b = lo;
+        if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
+          const uint32_t pre = (codes[lo] >> (shift + 3)) << (shift + 3);
+          b = lower_bound(codes, lo, cnt, pre | (k << shift));
+        }
+        s_b[j][k] = b;
+        if (k == 0) s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
+      }
+    }
+    __syncthreads();
+
+    // phase 2: one node per thread
+    const uint32_t local = tile * GS_BUILD_TILE + threadIdx.x;
+    const bool valid = local < count;
+    const uint32_t idx = start + local;
+    uint32_t mask = 0, nchild = 0;
+    bool leaf = true;
+    if (valid) {
+      const uint32_t hi = s_b[threadIdx.x][8];
+      leaf = hi == 0xffffffffu;
+      if (!leaf) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const uint32_t b0 = s_b[threadIdx.x][k];
+          const uint32_t b1 = k < 7 ? s_b[threadIdx.x][k + 1] : hi;
+          if (b1 > b0) mask |= 1u << k;
+        }
+        nchild = __popc(mask);
+      }
+    }
+    // block exclusive scan of child counts
+    uint32_t x = nchild;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[w] = x;
+    __syncthreads();
+    uint32_t woff = 0, tile_total = 0;
+    for (int k = 0; k < GS_BUILD_TILE / 64; k++) {
+      if (k < w) woff += s_wsum[k];
+      tile_total += s_wsum[k];
+    }
+    const uint32_t excl_local = woff + x - nchild;
+
+    if (threadIdx.x == 0) {
+      uint64_t* st = status + tile;
+      uint64_t prefix = 0;
+      if (tile == 0) {
+        st_agent64(st, kPre | tile_total);
+      } else {
+        st_agent64(st, kAgg | tile_total);
+        int32_t j = (int32_t)tile - 1;
+        uint32_t spins = 0;
+        for (;;) {
+          const uint64_t v = ld_agent64(status + j);
+          const uint64_t f = v & 0xffffffff00000000ull;
+          if (f == 0) {
+            if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
+              atomicOr(&sc->status, GS_ST_SPIN);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          prefix += v & 0xffffffffull;
+          if (f == kPre) break;
+          j--;
+        }
+        st_agent64(st, kPre | (prefix + tile_total));
+      }
+      s_prefix = prefix;
+      if (tile == ntiles - 1) {
+        uint64_t total = prefix + tile_total;
+        uint64_t cnt_next = total;
+        if ((uint64_t)next_start + total > t.cap) {
+          atomicOr(&sc->status, GS_ST_NODE_POOL);
+          cnt_next = t.cap > next_start ? t.cap - next_start : 0;
+        }
+        sc->level_start[d + 1] = next_start;
+        sc->level_count[d + 1] = (uint32_t)cnt_next;
+        sc->num_nodes = next_start + (uint32_t)cnt_next;
+        sc->num_levels = (uint32_t)d + 1;
+      }
+    }
+    __syncthreads();
+
+    if (valid) {
+      const int4 inf = t.info[idx];
+      if (leaf) {
+        t.info[idx] = make_int4((int)size_d, inf.y, inf.z, 1);
+      } else {
+        const uint32_t fc = next_start + (uint32_t)s_prefix + excl_local;
+        const uint32_t hi = s_b[threadIdx.x][8];
+        uint32_t c = fc;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          if (mask & (1u << k)) {
+            const uint32_t b0 = s_b[threadIdx.x][k];
+            const uint32_t b1 = k < 7 ? s_b[threadIdx.x][k + 1] : hi;
+            if (c < t.cap) {
+              t.info[c] = make_int4((int)size_c, (int)b0, (int)(b1 - b0), 0);
+              t.parent[c] = (int32_t)idx;
+            }
+            c++;
+          }
+        }
+        t.info[idx] = make_int4((int)size_d, (int)fc, (int)nchild, (int)(mask << 8));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+hipError_t gs_launch_build(gs_sim* s) {
+  uint32_t grid = s->build_tiles < 1024 ? s->build_tiles : 1024;
+  if (grid < 1) grid = 1;
+  for (int d = 0; d <= GS_MAX_LEVEL; d++) {
+    build_level_kernel<<<grid, GS_BUILD_TILE, 0, s->stream>>>(
+        s->keys[0], s->n, s->tree, s->sc, d, s->bucket, s->max_nodes, s->build_status[d & 1],
+        s->build_status[(d + 1) & 1], s->build_tiles);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// ---- expansion to the reference's 64-byte node layout (mortonTree.js:12-21)
+namespace {
+__global__ __launch_bounds__(256) void node64_kernel(DevTree t, const DevScalars* __restrict__ sc,
+                                                     uint32_t* __restrict__ out) {
+  const uint32_t num = sc->num_nodes;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
+    const float4 c = t.cm[i];
+    const int4 inf = t.info[i];
+    uint32_t w[16];
+    w[0] = __float_as_uint(c.x);
+    w[1] = __float_as_uint(c.y);
+    w[2] = __float_as_uint(c.z);
+    w[3] = __float_as_uint(c.w);
+    w[4] = (uint32_t)inf.x;
+    const bool leaf = inf.w & 1;
+    w[5] = leaf ? 1u : 0u;
+    w[6] = leaf ? (uint32_t)inf.y : 0u;
+    w[7] = leaf ? (uint32_t)inf.z : 0u;
+    const uint32_t mask = leaf ? 0u : ((uint32_t)inf.w >> 8) & 0xffu;
+    uint32_t c0 = (uint32_t)inf.y;
+    for (int k = 0; k < 8; k++) {
+      if (mask & (1u << k)) {
+        w[8 + k] = c0++;
+      } else {
+        w[8 + k] = 0xffffffffu;
+      }
+    }
+    uint4* o = (uint4*)(out + (size_t)i * 16);
+    o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    o[2] = make_uint4(w[8], w[9], w[10], w[11]);
+    o[3] = make_uint4(w[12], w[13], w[14], w[15]);
+  }
+}
+}  // namespace
+
+hipError_t gs_launch_node64(gs_sim* s) {
+  uint32_t grid = (s->tree.cap + 255) / 256;
+  if (grid > 4096) grid = 4096;
+  node64_kernel<<<grid, 256, 0, s->stream>>>(s->tree, s->sc, s->node64);
+  return hipGetLastError();
+}

@@ -1,0 +1,315 @@
+// Bounding box, root cube and 30-bit Morton encode (mortonTree.js:43-62).
+//
+//   bbox_kernel   per-block min/max over particles whose x+y+z is finite
+//                 (mortonTree.js:46-52); also clears the sort/build tickets,
+//                 digit histograms and onesweep status words of this step.
+//   root_kernel   one block: reduces the partials, then one lane evaluates
+//                 size = max(dx,dy,dz)*1.0001 + 1e-9 and s = 1024/size in
+//                 integer soft-double (bit-exact with the JS Number math),
+//                 the per-level node sizes f32(size * 2^-d), and the root
+//                 node's segment [0, N).
+//   encode_kernel q = min(ToInt32((p - min) * s), 1023) per axis, again in
+//                 soft-double, code = part1by2(qx) | qy<<1 | qz<<2; writes the
+//                 sort's (key, index) input and the 4 global digit histograms.
+// All three are HBM-streaming: 16 B read per particle for bbox, 16 B read +
+// 8 B written for encode.
+#include "gs_internal.h"
+#include "softdouble.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t part1by2(uint32_t v) {  // mortonTree.js:29-36
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x30000ffu;
+  v = (v | (v << 8)) & 0x300f00fu;
+  v = (v | (v << 4)) & 0x30c30c3u;
+  v = (v | (v << 2)) & 0x9249249u;
+  return v;
+}
+
+__device__ __forceinline__ bool finite3(float4 p) {
+  // JS isFinite(x + y + z) on f32 inputs summed in double: the sum of three
+  // finite f32 values never overflows binary64, so this is "all finite".
+  return __builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+__global__ __launch_bounds__(256) void reset_kernel(DevScalars* sc, uint32_t* sort_status,
+                                                    uint64_t status_words) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = g; k < status_words; k += stride) sort_status[k] = 0;
+  if (g < 32) sc->tickets[g] = 0;
+  for (uint64_t k = g; k < GS_SORT_PASSES * GS_RADIX; k += stride)
+    (&sc->sort_hist[0][0])[k] = 0;
+}
+
+__global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ posm, uint32_t n,
+                                                   float* __restrict__ part, DevScalars* sc,
+                                                   uint32_t* sort_status, uint64_t status_words) {
+  // this step's sort/build bookkeeping is cleared here, one launch ahead of use
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t gstride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = g; k < status_words; k += gstride) sort_status[k] = 0;
+  if (g < 32) sc->tickets[g] = 0;
+  for (uint64_t k = g; k < GS_SORT_PASSES * GS_RADIX; k += gstride)
+    (&sc->sort_hist[0][0])[k] = 0;
+
+  float mnx = INFINITY, mny = INFINITY, mnz = INFINITY;
+  float mxx = -INFINITY, mxy = -INFINITY, mxz = -INFINITY;
+  uint32_t cnt = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 p = posm[i];
+    if (finite3(p)) {
+      mnx = fminf(mnx, p.x); mxx = fmaxf(mxx, p.x);
+      mny = fminf(mny, p.y); mxy = fmaxf(mxy, p.y);
+      mnz = fminf(mnz, p.z); mxz = fmaxf(mxz, p.z);
+      cnt++;
+    }
+  }
+  mnx = wave_min(mnx); mny = wave_min(mny); mnz = wave_min(mnz);
+  mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  __shared__ float s[4][7];
+  int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    s[w][0] = mnx; s[w][1] = mny; s[w][2] = mnz;
+    s[w][3] = mxx; s[w][4] = mxy; s[w][5] = mxz;
+    s[w][6] = __uint_as_float(cnt);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float r[6] = {s[0][0], s[0][1], s[0][2], s[0][3], s[0][4], s[0][5]};
+    uint32_t c = __float_as_uint(s[0][6]);
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+      for (int j = 0; j < 3; j++) r[j] = fminf(r[j], s[k][j]);
+      for (int j = 3; j < 6; j++) r[j] = fmaxf(r[j], s[k][j]);
+      c += __float_as_uint(s[k][6]);
+    }
+    float* o = part + (size_t)blockIdx.x * 8;
+    for (int j = 0; j < 6; j++) o[j] = r[j];
+    o[6] = __uint_as_float(c);
+  }
+}
+
+__global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ part, uint32_t nparts,
+                                                   uint32_t n, DevScalars* sc, DevTree t) {
+  float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  uint32_t c = 0;
+  for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+    const float* o = part + (size_t)b * 8;
+    for (int j = 0; j < 3; j++) r[j] = fminf(r[j], o[j]);
+    for (int j = 3; j < 6; j++) r[j] = fmaxf(r[j], o[j]);
+    c += __float_as_uint(o[6]);
+  }
+  for (int j = 0; j < 3; j++) r[j] = wave_min(r[j]);
+  for (int j = 3; j < 6; j++) r[j] = wave_max(r[j]);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  __shared__ float s[4][7];
+  int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    for (int j = 0; j < 6; j++) s[w][j] = r[j];
+    s[w][6] = __uint_as_float(c);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+    for (int j = 0; j < 3; j++) r[j] = fminf(r[j], s[k][j]);
+    for (int j = 3; j < 6; j++) r[j] = fmaxf(r[j], s[k][j]);
+  }
+  c = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) c += __float_as_uint(s[k][6]);
+  sc->nfinite = c;
+  uint64_t size;
+  if (c == 0) {
+    // no finite particle: minX = +Inf, size = -Inf, s = -0 -> every code is 0
+    size = 0xfff0000000000000ull;
+    for (int j = 0; j < 3; j++) sc->min_bits[j] = 0x7ff0000000000000ull;
+    sc->s_bits = 0x8000000000000000ull;
+  } else {
+    uint64_t d[3];
+    for (int j = 0; j < 3; j++) {
+      uint64_t mn = sd::from_f32(__float_as_uint(r[j]));
+      uint64_t mx = sd::from_f32(__float_as_uint(r[j + 3]));
+      sc->min_bits[j] = mn;
+      d[j] = sd::sub(mx, mn);  // >= +0
+    }
+    uint64_t m = d[0];
+    if (sd::lt(m, d[1])) m = d[1];
+    if (sd::lt(m, d[2])) m = d[2];
+    const uint64_t k10001 = 0x3FF00068DB8BAC71ull;  // 1.0001
+    const uint64_t k1em9 = 0x3E112E0BE826D695ull;   // 1e-9
+    size = sd::add(sd::mul(m, k10001), k1em9);
+    sc->s_bits = sd::div(0x4090000000000000ull /* 1024.0 */, size);
+  }
+  sc->size_bits = size;
+  // node sizes per depth: f32(rootSize * 0.5^d) (mortonTree.js:124,161)
+  uint64_t sz = size;
+  for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
+    sc->level_size[dpt] = __uint_as_float(sd::to_f32(sz));
+    if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
+  }
+  // level 0 = the root, covering [0, N)
+  sc->level_start[0] = 0;
+  sc->level_count[0] = 1;
+  for (int k = 1; k <= GS_MAX_LEVELS; k++) {
+    sc->level_start[k] = 0;
+    sc->level_count[k] = 0;
+  }
+  sc->num_nodes = 1;
+  sc->num_levels = 0;
+  t.info[0] = make_int4((int)__float_as_uint(sc->level_size[0]), 0, (int)n, 0);
+  t.parent[0] = -1;
+}
+
+__device__ __forceinline__ uint32_t quant(float p, uint64_t mn, uint64_t s) {
+  // Math.min((p - min) * s | 0, 1023); non-finite p gives NaN/Inf -> 0
+  if (!__builtin_isfinite(p)) return 0;
+  uint64_t q = sd::mul(sd::sub(sd::from_f32(__float_as_uint(p)), mn), s);
+  int32_t v = sd::to_int32(q);
+  return (uint32_t)(v < 1023 ? v : 1023);  // part1by2 keeps the low 10 bits
+}
+
+template <bool KEEP_RAW>
+__global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ posm, uint32_t n,
+                                                     uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals,
+                                                     uint32_t* __restrict__ raw, DevScalars* sc) {
+  __shared__ uint32_t hist[GS_SORT_PASSES][GS_RADIX];
+  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) (&hist[0][0])[k] = 0;
+  __syncthreads();
+  const uint64_t mx = sc->min_bits[0], my = sc->min_bits[1], mz = sc->min_bits[2];
+  const uint64_t s = sc->s_bits;
+  const bool empty = sc->nfinite == 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 p = posm[i];
+    uint32_t code = 0;
+    if (!empty)
+      code = part1by2(quant(p.x, mx, s)) | (part1by2(quant(p.y, my, s)) << 1) |
+             (part1by2(quant(p.z, mz, s)) << 2);
+    keys[i] = code;
+    vals[i] = i;
+    if (KEEP_RAW) raw[i] = code;
+#pragma unroll
+    for (int ps = 0; ps < GS_SORT_PASSES; ps++)
+      atomicAdd(&hist[ps][(code >> (ps * GS_RADIX_BITS)) & (GS_RADIX - 1)], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) {
+    uint32_t v = (&hist[0][0])[k];
+    if (v) atomicAdd(&(&sc->sort_hist[0][0])[k], v);
+  }
+}
+
+// digit histograms of host-supplied keys (gs_radix_sort_indices path)
+__global__ __launch_bounds__(256) void hist_kernel(uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals, uint32_t n,
+                                                   DevScalars* sc) {
+  __shared__ uint32_t hist[GS_SORT_PASSES][GS_RADIX];
+  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) (&hist[0][0])[k] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t code = keys[i];
+    vals[i] = i;
+#pragma unroll
+    for (int ps = 0; ps < GS_SORT_PASSES; ps++)
+      atomicAdd(&hist[ps][(code >> (ps * GS_RADIX_BITS)) & (GS_RADIX - 1)], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) {
+    uint32_t v = (&hist[0][0])[k];
+    if (v) atomicAdd(&(&sc->sort_hist[0][0])[k], v);
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// AoS-8 particle records <-> SoA (posm, velp)
+__global__ __launch_bounds__(256) void pack_soa_kernel(const float4* __restrict__ aos, uint32_t n,
+                                                       float4* __restrict__ posm,
+                                                       float4* __restrict__ velp) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    posm[i] = aos[2 * (size_t)i];
+    velp[i] = aos[2 * (size_t)i + 1];
+  }
+}
+__global__ __launch_bounds__(256) void unpack_soa_kernel(float4* __restrict__ aos, uint32_t n,
+                                                         const float4* __restrict__ posm,
+                                                         const float4* __restrict__ velp) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    aos[2 * (size_t)i] = posm[i];
+    aos[2 * (size_t)i + 1] = velp[i];
+  }
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return (uint32_t)(g < cap ? g : cap);
+}
+
+}  // namespace
+
+hipError_t gs_launch_reset(gs_sim* s) {
+  uint64_t words = (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX;
+  reset_kernel<<<grid_for(words, 256, 2048), 256, 0, s->stream>>>(s->sc, s->sort_status, words);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_bbox(gs_sim* s, const float4* posm) {
+  uint64_t words = (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX;
+  bbox_kernel<<<GS_BBOX_BLOCKS, 256, 0, s->stream>>>(posm, s->n, s->bbox_part, s->sc,
+                                                     s->sort_status, words);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_root(gs_sim* s) {
+  root_kernel<<<1, 256, 0, s->stream>>>(s->bbox_part, GS_BBOX_BLOCKS, s->n, s->sc, s->tree);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_encode(gs_sim* s, const float4* posm, bool keep_raw) {
+  uint32_t g = grid_for(s->n, 256, 4096);
+  if (keep_raw)
+    encode_kernel<true><<<g, 256, 0, s->stream>>>(posm, s->n, s->keys[0], s->vals[0],
+                                                    s->codes_raw, s->sc);
+  else
+    encode_kernel<false><<<g, 256, 0, s->stream>>>(posm, s->n, s->keys[0], s->vals[0],
+                                                     s->codes_raw, s->sc);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_hist(gs_sim* s) {
+  hist_kernel<<<grid_for(s->n, 256, 4096), 256, 0, s->stream>>>(s->keys[0], s->vals[0], s->n,
+                                                                  s->sc);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st) {
+  fill_u32_kernel<<<grid_for(n, 256, 4096), 256, 0, st>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_pack_soa(gs_sim* s, const float* aos, int which) {
+  pack_soa_kernel<<<grid_for(s->n, 256, 4096), 256, 0, s->stream>>>(
+      (const float4*)aos, s->n, s->posm[which], s->velp[which]);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_unpack_soa(gs_sim* s, float* aos, int which) {
+  unpack_soa_kernel<<<grid_for(s->n, 256, 4096), 256, 0, s->stream>>>(
+      (float4*)aos, s->n, s->posm[which], s->velp[which]);
+  return hipGetLastError();
+}

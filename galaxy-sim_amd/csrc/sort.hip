@@ -1,0 +1,191 @@
+// Stable LSD radix sort of (Morton code, particle index) pairs — the device
+// form of radixSortIndices (mortonTree.js:68-84).  A stable sort by key of the
+// identity index array is unique, so 4 passes of 8-bit digits give exactly the
+// permutation of the reference's 3 passes of 10-bit digits.
+//
+// One kernel per pass ("onesweep"):
+//   1. a block takes the next 4096-key tile in launch order (atomic ticket);
+//   2. each wavefront ranks its 16 x 64 keys in order: per 64-key round, eight
+//      wave ballots give every lane the mask of lanes holding the same digit
+//      (peers), rank = running per-wave digit count + popcount(peers below);
+//   3. per digit, the tile publishes its count and looks back over earlier
+//      tiles' published counts / inclusive prefixes (decoupled look-back) to
+//      get its global output offset, added to the digit's global base from
+//      the histograms the encode kernel accumulated;
+//   4. keys are staged in LDS in tile-sorted order and written out as
+//      contiguous per-digit runs.
+// The last pass also gathers the particle records through the permutation
+// (apply_permutation, nbody_bh.wgsl:319-326), so sorted positions land in the
+// other ping-pong buffer without a separate pass over the index array.
+//
+// HBM per pass: 8 B read + 8 B written per key (+ 32 B gathered + 32 B
+// written per particle in the last pass).
+#include "gs_internal.h"
+
+namespace {
+
+constexpr uint32_t kAgg = 1u << 30;
+constexpr uint32_t kPre = 2u << 30;
+constexpr uint32_t kVal = (1u << 30) - 1;
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// exclusive scan over the 256 threads of a block (one value each)
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wsum[w] = x;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int k = 0; k < w; k++) off += s_wsum[k];
+  __syncthreads();
+  return off + x - v;
+}
+
+template <bool GATHER>
+__global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int pass,
+    uint32_t* status, DevScalars* sc, const float4* __restrict__ pin,
+    const float4* __restrict__ qin, float4* __restrict__ pout, float4* __restrict__ qout) {
+  constexpr int ITEMS = GS_SORT_ITEMS;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wh[GS_SORT_BLOCK / 64][GS_RADIX];
+  __shared__ uint32_t s_dstart[GS_RADIX];
+  __shared__ uint32_t s_gbase[GS_RADIX];
+  __shared__ uint32_t s_wsum[GS_SORT_BLOCK / 64];
+  __shared__ uint32_t s_k[GS_SORT_TILE];
+  __shared__ uint32_t s_v[GS_SORT_TILE];
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int shift = pass * GS_RADIX_BITS;
+  if (tid == 0) s_tile = atomicAdd(&sc->tickets[pass], 1u);
+  for (int k = tid; k < (GS_SORT_BLOCK / 64) * GS_RADIX; k += GS_SORT_BLOCK) (&s_wh[0][0])[k] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * GS_SORT_TILE;
+  const uint64_t below_mask = (1ull << lane) - 1;
+
+  uint32_t key[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    const uint32_t idx = base + (uint32_t)w * (64 * ITEMS) + (uint32_t)r * 64 + lane;
+    const bool valid = idx < n;
+    const uint32_t k = valid ? kin[idx] : 0u;
+    key[r] = k;
+    const uint32_t d = (k >> shift) & (GS_RADIX - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < GS_RADIX_BITS; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t below = (uint32_t)__popcll(peers & below_mask);
+    uint32_t prior = 0;
+    if (valid) prior = s_wh[w][d];
+    rank[r] = prior + below;
+    if (valid && below == 0) s_wh[w][d] = prior + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+
+  // per digit (one per thread): wave-exclusive offsets and the tile count
+  const uint32_t d = tid;
+  uint32_t c[GS_SORT_BLOCK / 64];
+  uint32_t total = 0;
+#pragma unroll
+  for (int k = 0; k < GS_SORT_BLOCK / 64; k++) {
+    c[k] = s_wh[k][d];
+    s_wh[k][d] = total;
+    total += c[k];
+  }
+  uint32_t* st = status + (size_t)tile * GS_RADIX + d;
+  st_agent(st, (tile == 0 ? kPre : kAgg) | total);
+
+  const uint32_t hist_excl = block_excl_scan256(sc->sort_hist[pass][d], s_wsum);
+  const uint32_t tile_excl = block_excl_scan256(total, s_wsum);
+
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int32_t j = (int32_t)tile - 1;
+    uint32_t spins = 0;
+    for (;;) {
+      const uint32_t v = ld_agent(status + (size_t)j * GS_RADIX + d);
+      const uint32_t f = v & ~kVal;
+      if (f == 0) {
+        if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
+          atomicOr(&sc->status, GS_ST_SPIN);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += v & kVal;
+      if (f == kPre) break;
+      j--;
+    }
+    st_agent(st, kPre | (excl + total));
+  }
+  s_gbase[d] = hist_excl + excl;
+  s_dstart[d] = tile_excl;
+  __syncthreads();
+
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    const uint32_t idx = base + (uint32_t)w * (64 * ITEMS) + (uint32_t)r * 64 + lane;
+    if (idx < n) {
+      const uint32_t dd = (key[r] >> shift) & (GS_RADIX - 1);
+      const uint32_t pos = s_dstart[dd] + s_wh[w][dd] + rank[r];
+      s_k[pos] = key[r];
+      s_v[pos] = vin[idx];
+    }
+  }
+  __syncthreads();
+
+  const uint32_t tile_n = n - base < (uint32_t)GS_SORT_TILE ? n - base : (uint32_t)GS_SORT_TILE;
+  for (uint32_t i = tid; i < tile_n; i += GS_SORT_BLOCK) {
+    const uint32_t k = s_k[i];
+    const uint32_t dd = (k >> shift) & (GS_RADIX - 1);
+    const uint32_t out = s_gbase[dd] + (i - s_dstart[dd]);
+    const uint32_t v = s_v[i];
+    if (out < n) {
+      kout[out] = k;
+      vout[out] = v;
+      if (GATHER && v < n) {
+        pout[out] = pin[v];
+        qout[out] = qin[v];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t gs_launch_sort(gs_sim* s, bool gather) {
+  const uint32_t tiles = (s->n + GS_SORT_TILE - 1) / GS_SORT_TILE;
+  for (int p = 0; p < GS_SORT_PASSES; p++) {
+    uint32_t* status = s->sort_status + (size_t)p * s->sort_tiles * GS_RADIX;
+    const int a = p & 1, b = a ^ 1;
+    if (gather && p == GS_SORT_PASSES - 1)
+      onesweep_kernel<true><<<tiles, GS_SORT_BLOCK, 0, s->stream>>>(
+          s->keys[a], s->vals[a], s->keys[b], s->vals[b], s->n, p, status, s->sc,
+          s->posm[s->cur], s->velp[s->cur], s->posm[s->cur ^ 1], s->velp[s->cur ^ 1]);
+    else
+      onesweep_kernel<false><<<tiles, GS_SORT_BLOCK, 0, s->stream>>>(
+          s->keys[a], s->vals[a], s->keys[b], s->vals[b], s->n, p, status, s->sc, nullptr,
+          nullptr, nullptr, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}

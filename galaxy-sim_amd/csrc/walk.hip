@@ -1,0 +1,267 @@
+// Barnes-Hut walk + integrate — the `main` kernel of nbody_bh.wgsl:64-181.
+//
+// Wavefront-coherent traversal with per-lane exactness.  The 64 lanes of a
+// wavefront own 64 consecutive Morton-sorted particles and share ONE stack in
+// LDS whose entries are (node, 64-bit lane mask).  Popping an entry, the
+// lanes in its mask evaluate the reference's own per-particle decision for
+// that node (nbody_bh.wgsl:97-141): monopole if s^2 < theta^2 d^2 or
+// s^2 < eps^2/4, else leaf direct sum (skipping j == i), else open.  The
+// children of an opened node are pushed with the mask of the lanes that
+// opened it, octant 0 first, so every lane visits exactly the nodes its own
+// per-thread stack would visit, in the same order: its interaction list and
+// its summation order equal the reference walk's.  The node index is
+// wave-uniform, so node and leaf-member loads are scalar (SMEM) broadcast
+// loads instead of 64 divergent vector loads, and divergence costs only the
+// lanes masked off.
+//
+// The shared stack never holds more than 7 * 9 + 8 = 71 entries (the DFS of
+// the union tree keeps <= 7 unvisited siblings per open level, depth <= 10).
+//
+// Arithmetic: the opening test uses the exact reference expression order
+// d2 = ((rx*rx + ry*ry) + rz*rz) + eps2 with no contraction (the library is
+// built with -ffp-contract=off), so decisions match the transliteration bit
+// for bit.  EXACT evaluates f = G*m / (d2*sqrt(d2)) with IEEE sqrt and
+// division (bit-identical forces); the default path uses the hardware
+// reciprocal square root, f = G*m * rsq(d2)^3.
+//
+// After the walk (STEP): Hernquist halos, kick-drift, gas drag, pad density
+// and specific energy (nbody_bh.wgsl:144-180), then per-block FP64 partial
+// sums of kinetic, pair-potential and halo energy, momentum, the reference's
+// specific-energy sum and mass (fixed-order, deterministic).
+#include "gs_internal.h"
+
+namespace {
+
+template <bool EXACT>
+__device__ __forceinline__ void interact(float gm, float rx, float ry, float rz, float d2,
+                                         float& ax, float& ay, float& az, float& pot) {
+  if (EXACT) {
+    const float f = gm / (d2 * __builtin_sqrtf(d2));
+    ax = ax + rx * f;
+    ay = ay + ry * f;
+    az = az + rz * f;
+    pot = pot - f * d2;
+  } else {
+    const float inv = __builtin_amdgcn_rsqf(d2);
+    const float gi = gm * inv;
+    const float f = gi * (inv * inv);
+    ax = ax + rx * f;
+    ay = ay + ry * f;
+    az = az + rz * f;
+    pot = pot - gi;
+  }
+}
+
+__device__ __forceinline__ uint64_t rfl64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <bool EXACT, bool STEP, bool COUNT>
+__global__ __launch_bounds__(1024) void walk_kernel(
+    const float4* __restrict__ posm, const float4* __restrict__ velp,
+    float4* __restrict__ posm_out, float4* __restrict__ velp_out, float* __restrict__ energy,
+    float4* __restrict__ acc_out, float* __restrict__ dens_out, uint32_t* __restrict__ inter_out,
+    const float4* __restrict__ ncm, const int4* __restrict__ ninfo,
+    const DevParams* __restrict__ prm, DevScalars* sc, double* __restrict__ red, uint32_t n) {
+  extern __shared__ uint64_t smem[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  uint64_t* stm = smem + (size_t)w * GS_WALK_STACK;
+  int32_t* stn = (int32_t*)(smem + (size_t)waves * GS_WALK_STACK) + (size_t)w * GS_WALK_STACK;
+
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
+  const float qeps2 = 0.25f * eps2;
+
+  float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
+  uint32_t cnt = 0;
+
+  const uint64_t all = __ballot(valid);
+  int sp = 0;
+  if (all) {
+    if (lane == 0) {
+      stn[0] = 0;
+      stm[0] = all;
+    }
+    sp = 1;
+  }
+  while (sp > 0) {
+    sp--;
+    const int32_t nd = __builtin_amdgcn_readfirstlane(stn[sp]);
+    const uint64_t msk = rfl64(stm[sp]);
+    const float4 c = ncm[nd];
+    const int4 inf = ninfo[nd];
+    if (c.w == 0.0f) continue;
+    const bool act = (msk >> lane) & 1ull;
+    const float size = __int_as_float(inf.x);
+    const bool leaf = (inf.w & 1) != 0;
+    const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane(inf.y);
+    const uint32_t count = (uint32_t)__builtin_amdgcn_readfirstlane(inf.z);
+    const float rx = c.x - p.x, ry = c.y - p.y, rz = c.z - p.z;
+    const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
+    const float s2 = size * size;
+    if (act && leaf && i - first < count) dens = (float)count / (s2 * size);
+    const bool mono = act && (s2 < theta2 * d2 || s2 < qeps2);
+    if (mono) {
+      interact<EXACT>(G * c.w, rx, ry, rz, d2, ax, ay, az, pot);
+      if (COUNT) cnt++;
+    }
+    const bool need = act && !mono;
+    if (leaf) {
+      if (__ballot(need)) {
+        for (uint32_t j = first; j < first + count; j++) {
+          const float4 q = posm[j];
+          if (need && j != i) {
+            const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
+            const float dj2 = ((qx * qx + qy * qy) + qz * qz) + eps2;
+            interact<EXACT>(G * q.w, qx, qy, qz, dj2, ax, ay, az, pot);
+            if (COUNT) cnt++;
+          }
+        }
+      }
+    } else {
+      const uint64_t om = __ballot(need);
+      if (om) {
+        const int nc = (int)count;  // internal: child count
+        if (sp + nc > GS_WALK_STACK) {
+          if (lane == 0) atomicOr(&sc->status, GS_ST_STACK);
+        } else {
+          if (lane < nc) {
+            stn[sp + lane] = (int32_t)first + lane;  // internal: first child
+            stm[sp + lane] = om;
+          }
+          sp += nc;
+        }
+      }
+    }
+  }
+
+  if (COUNT) {
+    if (valid) inter_out[i] = cnt;
+    uint64_t s = cnt;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0 && s) atomicAdd((unsigned long long*)&sc->interactions, (unsigned long long)s);
+  }
+
+  if (!STEP) {
+    if (valid) {
+      acc_out[i] = make_float4(ax, ay, az, pot);
+      dens_out[i] = dens;
+    }
+    return;
+  }
+
+  // ---- halos, kick-drift, gas drag, energy (nbody_bh.wgsl:144-180) --------
+  double r_ekin = 0, r_epot = 0, r_ehalo = 0, r_px = 0, r_py = 0, r_pz = 0, r_espec = 0, r_m = 0;
+  if (valid) {
+    const float4 v = velp[i];
+    float pot_halo = 0.f;
+    if (prm->halo_mass > 0.0f) {
+      const float gm = G * prm->halo_mass;
+      for (uint32_t h = 0; h < prm->halo_count; h++) {
+        const float4 ctr = h == 0 ? prm->halo_c1 : prm->halo_c2;
+        const float dx = p.x - ctr.x, dy = p.y - ctr.y, dz = p.z - ctr.z;
+        const float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
+        const float rlen = fmaxf(len, 1e-6f);
+        const float ra = rlen + prm->halo_a;
+        const float t = gm / ((ra * ra) * rlen);
+        ax = ax - dx * t;
+        ay = ay - dy * t;
+        az = az - dz * t;
+        pot_halo = pot_halo - gm / ra;
+      }
+    }
+    const float dt = prm->dt;
+    float vx = v.x + ax * dt, vy = v.y + ay * dt, vz = v.z + az * dt;
+    const bool gas = v.w < -0.5f;
+    if (gas && prm->gas_drag > 0.0f) {
+      float hx = -p.y, hy = p.x, hz = 1e-6f;
+      const float hl = __builtin_sqrtf((hx * hx + hy * hy) + hz * hz);
+      hx = hx / hl;
+      hy = hy / hl;
+      hz = hz / hl;
+      const float s = (vx * hx + vy * hy) + vz * hz;
+      const float fx = hx * s, fy = hy * s, fz = hz * s;
+      const float k = prm->drag_damp;
+      vx = fx + (vx - fx) * k;
+      vy = fy + (vy - fy) * k;
+      vz = fz + (vz - fz) * k;
+    }
+    posm_out[i] = make_float4(p.x + vx * dt, p.y + vy * dt, p.z + vz * dt, p.w);
+    velp_out[i] = make_float4(vx, vy, vz, gas ? -(1.0f + dens) : 0.0f);
+    const float v2 = (vx * vx + vy * vy) + vz * vz;
+    const float e = (0.5f * v2 + 0.5f * pot) + pot_halo;
+    energy[i] = e;
+    const double m = (double)p.w;
+    r_ekin = 0.5 * m * (double)v2;
+    r_epot = 0.5 * m * (double)pot;
+    r_ehalo = m * (double)pot_halo;
+    r_px = m * (double)vx;
+    r_py = m * (double)vy;
+    r_pz = m * (double)vz;
+    r_espec = (double)e;
+    r_m = m;
+  }
+  double acc[GS_NRED] = {r_ekin, r_epot, r_ehalo, r_px, r_py, r_pz, r_espec, r_m};
+  __shared__ double s_red[16][GS_NRED];
+#pragma unroll
+  for (int k = 0; k < GS_NRED; k++) acc[k] = wave_sum(acc[k]);
+  if (lane == 0)
+    for (int k = 0; k < GS_NRED; k++) s_red[w][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < GS_NRED) {
+    double s = 0;
+    for (int k = 0; k < waves; k++) s += s_red[k][threadIdx.x];
+    red[(size_t)blockIdx.x * GS_NRED + threadIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ red,
+                                                           uint32_t nblocks, double* out) {
+  __shared__ double s[4][GS_NRED];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a[GS_NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int k = 0; k < GS_NRED; k++) a[k] += red[(size_t)b * GS_NRED + k];
+  for (int k = 0; k < GS_NRED; k++) a[k] = wave_sum(a[k]);
+  if (lane == 0)
+    for (int k = 0; k < GS_NRED; k++) s[w][k] = a[k];
+  __syncthreads();
+  if (threadIdx.x < GS_NRED) out[threadIdx.x] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) +
+                                                 s[2][threadIdx.x]) + s[3][threadIdx.x];
+}
+
+template <bool EXACT, bool STEP, bool COUNT>
+hipError_t launch(gs_sim* s) {
+  const uint32_t wg = s->walk_wg;
+  const uint32_t grid = (s->n + wg - 1) / wg;
+  const size_t shm = (size_t)(wg / 64) * GS_WALK_STACK * (sizeof(uint64_t) + sizeof(int32_t));
+  const int c = s->cur;
+  walk_kernel<EXACT, STEP, COUNT><<<grid, wg, shm, s->stream>>>(
+      s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
+      s->inter, s->tree.cm, s->tree.info, s->prm, s->sc, s->red_part, s->n);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count, bool /*reduce*/) {
+  const bool exact = (s->flags & GS_FLAG_EXACT_FORCE) != 0;
+  if (count) return exact ? launch<true, false, true>(s) : launch<false, false, true>(s);
+  if (integrate) return exact ? launch<true, true, false>(s) : launch<false, true, false>(s);
+  return exact ? launch<true, false, false>(s) : launch<false, false, false>(s);
+}
+
+hipError_t gs_launch_reduce_final(gs_sim* s, double* out_dev) {
+  reduce_final_kernel<<<1, 256, 0, s->stream>>>(s->red_part, s->red_blocks, out_dev);
+  return hipGetLastError();
+}

@@ -1,0 +1,155 @@
+"""GPU parity of the Morton encode, stable sort, gather, BFS build and FP64
+refresh against the golden vectors written from the reference's
+mortonTree.js, and against the oracle restatement at sizes the fixtures do
+not cover.  All calls go through the C ABI (libgalaxy_hip.so).  Bit-exact."""
+import numpy as np
+import pytest
+
+import cases
+import galaxy_hip
+from galaxy_hip import ic
+from gpu_util import banner, oracle_tree
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _sorted_tree_case(case, data):
+    n = case["N"]
+    sim = galaxy_hip.NBodyGPU(data.reshape(n, 8), bucket=16, max_nodes=case["maxNodes"])
+    try:
+        sim.encode()
+        codes = sim.codes(sorted=False)
+        sim.sort()
+        perm = sim.perm()
+        sorted_ = sim.particles()
+        scodes = sim.codes(sorted=True)
+        info = sim.tree_info()
+        sim.refresh()
+        nodes = sim.nodes()
+    finally:
+        sim.destroy()
+    return codes, perm, sorted_, scodes, info, nodes
+
+
+@pytest.mark.parametrize("case", [c for c in cases.cases("stored") if not c.get("codesOnly")],
+                         ids=lambda c: c["name"])
+def test_tree_matches_mortonTree_js_golden(case):
+    banner(f"golden {case['name']} N={case['N']}")
+    data = cases.input_of(case)
+    codes, perm, sorted_, scodes, info, nodes = _sorted_tree_case(case, data)
+    name = case["name"]
+    assert np.array_equal(codes, cases.load(name, "codes.u32", np.uint32))
+    assert np.float64(info["root_size"]).tobytes().hex() == case["rootSizeHex"]
+    assert np.array_equal(perm, cases.load(name, "perm.u32", np.uint32))
+    assert cases.sha(sorted_.reshape(-1)) == case["sha"]["sorted"]
+    assert np.array_equal(scodes, codes[perm])
+    assert info["num_nodes"] == case["numNodes"]
+    assert info["level_ranges"] == case["levelRanges"]
+    want = cases.load(name, "nodes.u32", np.uint32)
+    if not np.array_equal(nodes, want):
+        bad = np.nonzero(nodes != want)[0]
+        raise AssertionError(f"{bad.size} node words differ, first at node {bad[0] // 16} "
+                             f"word {bad[0] % 16}: {nodes[bad[0]]:#x} vs {want[bad[0]]:#x}")
+
+
+@pytest.mark.parametrize("case", cases.cases("digest"), ids=lambda c: c["name"])
+def test_tree_matches_mortonTree_js_digests(case):
+    banner(f"digest {case['name']} N={case['N']}")
+    data = cases.input_of(case)
+    codes, perm, sorted_, scodes, info, nodes = _sorted_tree_case(case, data)
+    assert cases.sha(codes) == case["sha"]["codes"]
+    assert cases.sha(perm) == case["sha"]["perm"]
+    assert cases.sha(sorted_.reshape(-1)) == case["sha"]["sorted"]
+    assert info["num_nodes"] == case["numNodes"]
+    assert info["level_ranges"] == case["levelRanges"]
+    assert cases.sha(nodes) == case["sha"]["nodes"]
+
+
+def test_stride4_codes_host_surface():
+    case = next(c for c in cases.cases("stored") if c["name"] == "stride4_2k")
+    banner("stride4 codes")
+    data = cases.input_of(case)
+    codes, size = galaxy_hip.compute_morton_codes(data, case["N"], 4)
+    assert np.array_equal(codes, cases.load("stride4_2k", "codes.u32", np.uint32))
+    assert np.float64(size).tobytes().hex() == case["rootSizeHex"]
+
+
+@pytest.mark.parametrize("n", [1, 5, 4095, 4096, 4097, 100003, 1 << 20])
+def test_radix_sort_is_stable_sort(n):
+    banner(f"radix sort n={n}")
+    rng = np.random.default_rng(n)
+    codes = rng.integers(0, 1 << 30, n, dtype=np.uint32)
+    if n > 10:
+        codes[: n // 4] = codes[n // 4: 2 * (n // 4)]
+        codes[-(n // 8):] = 7  # long run of equal keys across tiles
+    perm = galaxy_hip.radix_sort_indices(codes)
+    assert np.array_equal(perm, np.argsort(codes, kind="stable").astype(np.uint32))
+
+
+@pytest.mark.parametrize("name,n,bucket", [("cpu_disk_16k", 16384, 16),
+                                           ("collision_4m", 200000, 32),
+                                           ("disk_500k", 500000, 16)])
+def test_tree_matches_oracle(name, n, bucket):
+    banner(f"oracle tree {name} n={n} bucket={bucket}")
+    data = ic.make(name, n=n)
+    ref = oracle_tree(data, bucket=bucket)
+    sim = galaxy_hip.NBodyGPU(data, bucket=bucket)
+    try:
+        sim.sort()
+        perm = sim.perm()
+        info = sim.tree_info()
+        sim.refresh()
+        nodes = sim.nodes()
+        sorted_ = sim.particles()
+    finally:
+        sim.destroy()
+    assert np.array_equal(perm, ref["perm"])
+    assert np.array_equal(sorted_.reshape(-1), ref["sorted"])
+    assert info["num_nodes"] == ref["num_nodes"]
+    assert info["level_ranges"] == ref["level_ranges"]
+    assert np.array_equal(nodes, ref["nodes"])
+
+
+def test_tree_build_host_surface_bucket32():
+    banner("MortonTree.build bucket 32")
+    data = ic.make("collision_4m", n=50000)
+    ref = oracle_tree(data, bucket=32)
+    srt = ref["sorted"].reshape(-1, 8)
+    t = galaxy_hip.MortonTree(50000, bucket=32).build(ref["codes"], srt[:, 0:3].copy(),
+                                                      srt[:, 3].copy(), 50000, ref["root_size"])
+    assert t.numNodes == ref["num_nodes"]
+    assert t.levelRanges == ref["level_ranges"]
+    assert np.array_equal(t.words, ref["nodes"])
+
+
+def test_refresh_after_motion_matches_fp64_oracle():
+    """COMs re-summed from moved particles equal the FP64 build-order sums."""
+    banner("refresh after motion")
+    data = ic.make("cpu_disk_16k")
+    sim = galaxy_hip.NBodyGPU(data, bucket=16)
+    try:
+        sim.update_params(dt=0.005, G=1.0, epsilon=0.02, theta=0.7)
+        sim.sort()
+        sim.step(2, sort_every=0)  # topology fixed, particles moved
+        sim.refresh()
+        nodes = sim.nodes()
+        parts = sim.particles()
+    finally:
+        sim.destroy()
+    want = oracle.refresh_f64(parts.reshape(-1), nodes)
+    assert np.array_equal(nodes, want)
+
+
+def test_tiny_pool_makes_the_root_a_leaf():
+    """maxNodes = 5: the pool rule (idx + 9 > maxNodes) leaves only the root."""
+    banner("tiny max_nodes")
+    data = ic.make("cpu_disk_16k", n=4096)
+    sim = galaxy_hip.NBodyGPU(data, bucket=16, max_nodes=5)
+    try:
+        sim.sort()
+        info = sim.tree_info()
+    finally:
+        sim.destroy()
+    ref = oracle_tree(data, bucket=16, max_nodes=5)
+    assert info["num_nodes"] == ref["num_nodes"] == 1

@@ -1,0 +1,196 @@
+"""GPU parity of the tree walk and integrator (nbody_bh.wgsl `main`) against
+the FP32 transliteration in the oracle, through the C ABI.
+
+EXACT mode (IEEE sqrt and division in every interaction) must be bit-identical
+to the transliteration: accelerations, potential, density, integrated
+particles and specific energies.  The default rsqrt path must stay within
+the stated force tolerance, 1e-5 relative per particle."""
+import numpy as np
+import pytest
+
+import galaxy_hip
+from galaxy_hip import ic
+from gpu_util import banner, oracle_tree, rel_err
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+FAST_TOL = 1e-5  # relative force error of the default (rsqrt) path
+
+
+def _params(n, **kw):
+    base = dict(dt=0.005, G=1.0, epsilon=0.02, theta=0.7)
+    base.update(kw)
+    halo = base.pop("haloCenters", ())
+    op = oracle.pack_params(n, dt=base["dt"], G=base["G"], epsilon=base["epsilon"],
+                            theta=base["theta"], halo_mass=base.get("haloMass", 0.0),
+                            halo_a=base.get("haloA", 1.0), gas_drag=base.get("gasDrag", 0.0),
+                            halo_centers=halo)
+    return base, halo, op
+
+
+def _gpu_force(data, bucket, exact, **kw):
+    base, halo, _ = _params(data.shape[0], **kw)
+    sim = galaxy_hip.NBodyGPU(data, bucket=bucket, exact_force=exact)
+    try:
+        sim.update_params(haloCenters=halo, **base)
+        sim.sort()
+        sim.refresh()
+        sim.force()
+        acc, dens = sim.accel()
+        sorted_ = sim.particles()
+    finally:
+        sim.destroy()
+    return acc, dens, sorted_
+
+
+@pytest.mark.parametrize("theta,eps", [(0.7, 0.02), (0.5, 0.005), (0.0, 0.02)])
+def test_exact_force_bit_identical(theta, eps):
+    banner(f"exact force theta={theta} eps={eps}")
+    data = ic.make("cpu_disk_16k")
+    ref = oracle_tree(data, bucket=16)
+    acc, dens, sorted_ = _gpu_force(data, 16, True, theta=theta, epsilon=eps)
+    assert np.array_equal(sorted_.reshape(-1), ref["sorted"])
+    _, _, op = _params(data.shape[0], theta=theta, epsilon=eps)
+    _, _, racc, rdens, _ = oracle.walk(ref["sorted"], ref["nodes"], op)
+    assert np.array_equal(acc.view(np.uint32), racc.view(np.uint32))
+    assert np.array_equal(dens.view(np.uint32), rdens.view(np.uint32))
+
+
+def test_fast_force_within_tolerance():
+    banner("fast force tolerance")
+    data = ic.make("cpu_disk_16k")
+    ref = oracle_tree(data, bucket=16)
+    acc, dens, _ = _gpu_force(data, 16, False, theta=0.7, epsilon=0.02)
+    _, _, op = _params(data.shape[0], theta=0.7, epsilon=0.02)
+    _, _, racc, rdens, _ = oracle.walk(ref["sorted"], ref["nodes"], op)
+    err = rel_err(acc[:, :3], racc[:, :3])
+    banner(f"fast path: max rel err {err.max():.3e}, p99 {np.quantile(err, 0.99):.3e}")
+    assert err.max() <= FAST_TOL
+    assert np.array_equal(dens, rdens)
+
+
+def _oracle_steps(data, nsteps, bucket, op):
+    cur = data.reshape(-1).copy()
+    energy = None
+    for _ in range(nsteps):
+        ref = oracle.sort_and_build(cur, data.shape[0], bucket=bucket)
+        pout, energy, _, _, _ = oracle.walk(ref["sorted"], ref["nodes"], op)
+        cur = pout
+    return cur, energy
+
+
+@pytest.mark.parametrize("name,n,extra", [
+    ("cpu_disk_16k", 16384, {}),
+    ("disk_500k", 30000, dict(haloMass=5.0, haloA=3.0, gasDrag=0.25,
+                              haloCenters=[(0.01, -0.02, 0.0)])),
+    ("collision_4m", 40000, dict(theta=0.5, epsilon=0.005, haloMass=1.0, haloA=2.0,
+                                 haloCenters=[(-4.0, 0.0, 0.0), (4.0, 0.0, 0.0)])),
+])
+def test_exact_steps_bit_identical(name, n, extra):
+    """5 full steps (sort + refresh + walk + kick-drift) equal the oracle's."""
+    banner(f"exact steps {name} n={n}")
+    data = ic.make(name, n=n)
+    bucket = 32 if name == "collision_4m" else 16
+    base, halo, op = _params(n, **extra)
+    sim = galaxy_hip.NBodyGPU(data, bucket=bucket, exact_force=True)
+    try:
+        sim.update_params(haloCenters=halo, **base)
+        sim.step(5, sort_every=1)
+        parts = sim.particles()
+        energy = sim.energy()
+        red = sim.reduce()
+    finally:
+        sim.destroy()
+    want, went = _oracle_steps(data, 5, bucket, op)
+    assert np.array_equal(parts.reshape(-1).view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(energy.view(np.uint32), went.view(np.uint32))
+    # FP64 reductions of the last step
+    p = parts.astype(np.float64)
+    m = p[:, 3]
+    v2 = (p[:, 4] ** 2 + p[:, 5] ** 2) + p[:, 6] ** 2
+    assert red["mass"] == pytest.approx(m.sum(), rel=1e-12)
+    assert red["px"] == pytest.approx((m * p[:, 4]).sum(), rel=1e-9, abs=1e-12)
+    assert red["espec"] == pytest.approx(energy.astype(np.float64).sum(), rel=1e-12)
+    assert red["ekin"] == pytest.approx(0.5 * (m * v2).sum(), rel=1e-6)
+
+
+def test_interaction_count_matches_oracle():
+    banner("interaction count")
+    data = ic.make("cpu_disk_16k")
+    ref = oracle_tree(data, bucket=16)
+    base, halo, op = _params(data.shape[0], theta=0.7, epsilon=0.02)
+    sim = galaxy_hip.NBodyGPU(data, bucket=16)
+    try:
+        sim.update_params(**base)
+        sim.sort()
+        sim.refresh()
+        total, per = sim.count_interactions(per_particle=True)
+    finally:
+        sim.destroy()
+    _, _, _, _, inter = oracle.walk(ref["sorted"], ref["nodes"], op)
+    assert np.array_equal(per, inter)
+    assert total == int(inter.astype(np.uint64).sum())
+
+
+def test_pair_force_closed_form():
+    """Two particles in one leaf: a = G m r / (r^2 + eps^2)^{3/2}."""
+    banner("pair closed form")
+    data = np.zeros((2, 8), np.float32)
+    data[0, :4] = (0.0, 0.0, 0.0, 2.0)
+    data[1, :4] = (3.0, 4.0, 0.0, 0.5)
+    acc, _, srt = _gpu_force(data, 16, True, theta=0.5, epsilon=0.1)
+    for i in range(2):
+        j = 1 - i
+        r = srt[j, :3].astype(np.float64) - srt[i, :3]
+        d2 = (r ** 2).sum() + 0.1 ** 2
+        want = srt[j, 3] * r / d2 ** 1.5
+        np.testing.assert_allclose(acc[i, :3], want, rtol=2e-7)
+
+
+def test_far_node_is_its_monopole():
+    """A distant cluster enters as one node: a = G M (com - x) / (d^2 + eps^2)^{3/2}."""
+    banner("far monopole closed form")
+    rng = np.random.default_rng(3)
+    n = 200
+    data = np.zeros((n + 1, 8), np.float32)
+    data[:n, :3] = rng.uniform(-0.01, 0.01, (n, 3)) + np.array([100.0, 0, 0])
+    data[:n, 3] = rng.uniform(0.5, 1.5, n)
+    data[n, :4] = (0.0, 0.0, 0.0, 1.0)
+    acc, _, srt = _gpu_force(data, 16, True, theta=0.5, epsilon=0.01)
+    k = int(np.nonzero(srt[:, 0] < 50)[0][0])
+    others = np.arange(n + 1) != k
+    M = srt[others, 3].astype(np.float64).sum()
+    com = (srt[others, :3].astype(np.float64) * srt[others, 3:4]).sum(0) / M
+    r = com - srt[k, :3]
+    want = M * r / ((r ** 2).sum() + 1e-4) ** 1.5
+    np.testing.assert_allclose(acc[k, :3], want, rtol=1e-5)
+
+
+def test_theta_zero_converges_to_fp64_direct_sum():
+    banner("theta 0 vs FP64 direct sum")
+    data = ic.make("cpu_disk_16k", n=4096)
+    acc, _, srt = _gpu_force(data, 16, False, theta=0.0, epsilon=0.02)
+    idx = np.arange(0, 4096, 7, dtype=np.uint32)
+    ds = oracle.direct_sum_f64(srt.reshape(-1), 1.0, np.float64(np.float32(0.02 * 0.02)), idx)
+    err = rel_err(acc[idx, :3], ds)
+    # theta = 0 still takes sub-softening nodes (s < eps/2) as monopoles
+    assert np.median(err) < 1e-5
+    assert err.max() < 1e-3
+
+
+def test_hernquist_halo_closed_form():
+    banner("halo closed form")
+    data = np.zeros((1, 8), np.float32)
+    data[0, :4] = (3.0, 0.0, 4.0, 1.0)
+    sim = galaxy_hip.NBodyGPU(data, exact_force=True)
+    try:
+        sim.update_params(dt=1e-3, G=1.0, epsilon=0.1, theta=0.5, haloMass=5.0, haloA=2.0,
+                          haloCenters=[(0.0, 0.0, 0.0)])
+        sim.step(1, sort_every=1)
+        p = sim.particles()
+    finally:
+        sim.destroy()
+    r = np.array([3.0, 0.0, 4.0])
+    a = -5.0 * r / (5.0 * (5.0 + 2.0) ** 2)
+    np.testing.assert_allclose(p[0, 4:7], a * 1e-3, rtol=1e-6)
