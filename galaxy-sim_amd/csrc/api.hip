@@ -382,6 +382,63 @@ int gs_update_central_mass(gs_sim* s, float mass, uint32_t idx) {
   });
 }
 
+int gs_apply_permutation(gs_sim* s, const uint32_t* perm, uint64_t n) {
+  return guarded(s, [&] {
+    need(perm != nullptr && n == s->n, GS_E_INVALID, "permutation needs N entries");
+    std::vector<uint8_t> seen(n, 0);
+    for (uint64_t i = 0; i < n; i++) {
+      need(perm[i] < n && !seen[perm[i]], GS_E_INVALID, "not a permutation of [0, N)");
+      seen[perm[i]] = 1;
+    }
+    check(hipMemcpyAsync(s->vals[0], perm, 4 * n, hipMemcpyHostToDevice, s->stream), "perm");
+    check(gs_launch_gather(s, s->vals[0]), "gather");
+    check(hipStreamSynchronize(s->stream), "gather");
+    s->cur ^= 1;
+    s->have_force = false;
+  });
+}
+
+int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
+                   const uint32_t* level_ranges, uint32_t num_levels) {
+  return guarded(s, [&] {
+    need(nodes != nullptr && num_nodes >= 1 && num_nodes <= s->tree.cap, GS_E_INVALID,
+         "node count outside [1, pool]");
+    need(num_levels <= GS_MAX_LEVELS && (num_levels == 0 || level_ranges), GS_E_INVALID,
+         "bad level ranges");
+    // host validation: every index the device will follow stays in bounds
+    for (uint64_t i = 0; i < num_nodes; i++) {
+      const uint32_t* w = nodes + i * 16;
+      if (w[5] == 1u) {
+        need(w[6] <= s->n && w[7] <= s->n - w[6], GS_E_INVALID, "leaf range outside [0, N)");
+      } else {
+        for (int k = 0; k < 8; k++) {
+          const int32_t c = (int32_t)w[8 + k];
+          need(c < 0 || ((uint64_t)c > i && (uint64_t)c < num_nodes), GS_E_INVALID,
+               "child index must follow its parent and lie inside the tree");
+        }
+      }
+    }
+    if (!s->node64) dalloc(&s->node64, (size_t)s->tree.cap * 16, "node staging");
+    check(hipMemcpyAsync(s->node64, nodes, num_nodes * 64, hipMemcpyHostToDevice, s->stream),
+          "nodes");
+    DevScalars h;
+    check(hipMemcpyAsync(&h, s->sc, sizeof h, hipMemcpyDeviceToHost, s->stream), "scalars");
+    check(hipStreamSynchronize(s->stream), "sync");
+    h.num_nodes = (uint32_t)num_nodes;
+    h.num_levels = num_levels;
+    for (uint32_t k = 0; k <= GS_MAX_LEVELS; k++) {
+      h.level_start[k] = k < num_levels ? level_ranges[2 * k] : 0;
+      h.level_count[k] = k < num_levels ? level_ranges[2 * k + 1] : 0;
+    }
+    h.status = 0;
+    check(hipMemcpyAsync(s->sc, &h, sizeof h, hipMemcpyHostToDevice, s->stream), "scalars");
+    check(gs_launch_import64(s, s->node64, (uint32_t)num_nodes), "import");
+    check_status(s);  // non-contiguous children are rejected here
+    s->have_tree = true;
+    s->have_force = false;
+  });
+}
+
 int gs_encode(gs_sim* s) {
   return guarded(s, [&] {
     Stage t(s, GS_T_ENCODE);

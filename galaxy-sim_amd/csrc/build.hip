@@ -261,3 +261,63 @@ hipError_t gs_launch_node64(gs_sim* s) {
   node64_kernel<<<grid, 256, 0, s->stream>>>(s->tree, s->sc, s->node64);
   return hipGetLastError();
 }
+
+// ---- import of a tree in the reference's 64-byte layout (uploadBHTree) ----
+namespace {
+__global__ __launch_bounds__(256) void import64_kernel(DevTree t, DevScalars* sc,
+                                                       const uint32_t* __restrict__ in,
+                                                       uint32_t num) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
+    const uint4* r = (const uint4*)(in + (size_t)i * 16);
+    const uint4 a = r[0], b = r[1], c = r[2], d = r[3];
+    t.cm[i] = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
+                          __uint_as_float(a.w));
+    if (i == 0) t.parent[0] = -1;
+    if (b.y == 1u) {
+      t.info[i] = make_int4((int)b.x, (int)b.z, (int)b.w, 1);
+      continue;
+    }
+    const uint32_t ch[8] = {c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
+    uint32_t mask = 0, fc = 0xffffffffu, nc = 0;
+    for (int k = 0; k < 8; k++) {
+      if ((int32_t)ch[k] >= 0) {
+        if (fc == 0xffffffffu) fc = ch[k];
+        // BFS trees keep a node's children contiguous in octant order
+        if (ch[k] != fc + nc) atomicOr(&sc->status, GS_ST_NODE_POOL);
+        mask |= 1u << k;
+        nc++;
+        if (ch[k] < num) t.parent[ch[k]] = (int32_t)i;
+      }
+    }
+    t.info[i] = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)nc, (int)(mask << 8));
+  }
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict__ perm, uint32_t n,
+                                                     const float4* __restrict__ pin,
+                                                     const float4* __restrict__ qin,
+                                                     float4* __restrict__ pout,
+                                                     float4* __restrict__ qout) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t s = perm[i];
+    pout[i] = pin[s];
+    qout[i] = qin[s];
+  }
+}
+}  // namespace
+
+hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, uint32_t num) {
+  uint32_t grid = (num + 255) / 256;
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  import64_kernel<<<grid, 256, 0, s->stream>>>(s->tree, s->sc, words, num);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_gather(gs_sim* s, const uint32_t* perm) {
+  uint32_t grid = (s->n + 255) / 256;
+  if (grid > 8192) grid = 8192;
+  gather_kernel<<<grid, 256, 0, s->stream>>>(perm, s->n, s->posm[s->cur], s->velp[s->cur],
+                                              s->posm[s->cur ^ 1], s->velp[s->cur ^ 1]);
+  return hipGetLastError();
+}

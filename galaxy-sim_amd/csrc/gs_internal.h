@@ -140,6 +140,8 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm);
 hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count, bool reduce);
 hipError_t gs_launch_reduce_final(gs_sim* s, double* out_dev);
 hipError_t gs_launch_node64(gs_sim* s);
+hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, uint32_t num);
+hipError_t gs_launch_gather(gs_sim* s, const uint32_t* perm);
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t gs_launch_pack_soa(gs_sim* s, const float* aos, int which);
 hipError_t gs_launch_unpack_soa(gs_sim* s, float* aos, int which);

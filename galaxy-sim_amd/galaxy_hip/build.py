@@ -65,7 +65,8 @@ def build_addon():
         return None
     cxx = shutil.which("g++") or "g++"
     tmp = ADDON + ".tmp"
-    _run([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-DNAPI_VERSION=8", "-I", inc,
+    _run([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-DNAPI_VERSION=8", "-DNODE_GYP_MODULE_NAME=galaxy_hip", "-Wall",
+          "-I", inc,
           "-I", os.path.join(ROOT, "include"), ADDON_SRC, "-o", tmp,
           "-L", LIBDIR, "-lgalaxy_hip", "-Wl,-rpath,$ORIGIN/../lib"])
     os.replace(tmp, ADDON)

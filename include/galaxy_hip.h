@@ -126,6 +126,16 @@ int gs_update_central_mass(gs_sim* sim, float mass, uint32_t idx);
 /* computeMortonCodes(pos, N, 8, codes) on the current particles
  * (mortonTree.js:43-62); keeps the codes for gs_download_codes(.., 0). */
 int gs_encode(gs_sim* sim);
+/* applyPermutation(enc, perm) (NBodyGPU.js:225-238): gather the particle
+ * records through a host permutation (perm[new] = old, a bijection of
+ * [0, N)) into the other ping-pong buffer. */
+int gs_apply_permutation(gs_sim* sim, const uint32_t* perm, uint64_t n);
+/* uploadBHTree(treeBuffer, numNodes, levelRanges) (NBodyGPU.js:288-301): a
+ * tree in the 64-byte layout built on the host (e.g. by mortonTree.js) for the
+ * current particle order.  Children of each node must be contiguous in octant
+ * order, as every breadth-first MortonTree.build emits them. */
+int gs_upload_tree(gs_sim* sim, const uint32_t* nodes, uint64_t num_nodes,
+                   const uint32_t* level_ranges, uint32_t num_levels);
 /* One topology rebuild: the worker's computeMortonCodes + radixSortIndices +
  * gather (bhWorker.js:58-80), then applyPermutation (NBodyGPU.js:225-238)
  * and MortonTree.build + uploadBHTree (mortonTree.js:107-182,
