@@ -120,10 +120,7 @@ void free_all(gs_sim* s) {
   f(s->codes_raw);
   f(s->sort_status);
   f(s->bbox_part);
-  f(s->tree.cm);
-  f(s->tree.info);
-  f(s->tree.parent);
-  f(s->tree.arrivals);
+  f(s->tree.node);
   f(s->energy);
   f(s->acc);
   f(s->dens);
@@ -292,10 +289,7 @@ int gs_create(const gs_config* cfg, gs_sim** out) {
     dalloc(&s->codes_raw, n, "codes");
     dalloc(&s->sort_status, (size_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX, "sort status");
     dalloc(&s->bbox_part, (size_t)GS_BBOX_BLOCKS * 8, "bbox");
-    dalloc(&s->tree.cm, cap, "node com");
-    dalloc(&s->tree.info, cap, "node info");
-    dalloc(&s->tree.parent, cap, "node parent");
-    dalloc(&s->tree.arrivals, cap, "node arrivals");
+    dalloc(&s->tree.node, cap, "nodes");
     dalloc(&s->energy, n, "energy");
     dalloc(&s->acc, n, "acc");
     dalloc(&s->dens, n, "density");
@@ -305,7 +299,6 @@ int gs_create(const gs_config* cfg, gs_sim** out) {
     dalloc(&s->prm, 1, "params");
     // zero state with fill kernels (graph-safe, no memset nodes)
     check(gs_launch_fill_u32((uint32_t*)s->sc, 0, sizeof(DevScalars) / 4, s->stream), "fill");
-    check(gs_launch_fill_u32(s->tree.arrivals, 0, cap, s->stream), "fill");
     check(gs_launch_fill_u32(s->energy ? (uint32_t*)s->energy : nullptr, 0, n, s->stream), "fill");
     for (int k = 0; k < 2; k++)
       check(gs_launch_fill_u32(s->build_status[k] ? (uint32_t*)s->build_status[k] : nullptr, 0,
@@ -719,9 +712,9 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     int4 root = make_int4(0, 0, (int)n, 0);
     float f0 = h.level_size[0];
     memcpy(&root.x, &f0, 4);
-    int32_t par = -1;
-    check(hipMemcpyAsync(s->tree.info, &root, sizeof root, hipMemcpyHostToDevice, s->stream), "root");
-    check(hipMemcpyAsync(s->tree.parent, &par, 4, hipMemcpyHostToDevice, s->stream), "root");
+    check(hipMemcpyAsync(&s->tree.node[0].info, &root, sizeof root, hipMemcpyHostToDevice,
+                         s->stream),
+          "root");
     check(gs_launch_build(s), "build");
     check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
     s->have_tree = true;

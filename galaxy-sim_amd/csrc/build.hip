@@ -86,7 +86,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
       const uint32_t local = tile * GS_BUILD_TILE + j;
       if (local < count) {
         const uint32_t idx = start + local;
-        const int4 inf = t.info[idx];
+        const int4 inf = t.node[idx].info;
         const uint32_t lo = (uint32_t)inf.y, cnt = (uint32_t)inf.z;
         const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
         uint32_t b = lo;
@@ -178,9 +178,9 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     __syncthreads();
 
     if (valid) {
-      const int4 inf = t.info[idx];
+      const int4 inf = t.node[idx].info;
       if (leaf) {
-        t.info[idx] = make_int4((int)size_d, inf.y, inf.z, 1);
+        t.node[idx].info = make_int4((int)size_d, inf.y, inf.z, 1);
       } else {
         const uint32_t fc = next_start + (uint32_t)s_prefix + excl_local;
         const uint32_t hi = s_b[threadIdx.x][8];
@@ -191,13 +191,12 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
             const uint32_t b0 = s_b[threadIdx.x][k];
             const uint32_t b1 = k < 7 ? s_b[threadIdx.x][k + 1] : hi;
             if (c < t.cap) {
-              t.info[c] = make_int4((int)size_c, (int)b0, (int)(b1 - b0), 0);
-              t.parent[c] = (int32_t)idx;
+              t.node[c].info = make_int4((int)size_c, (int)b0, (int)(b1 - b0), 0);
             }
             c++;
           }
         }
-        t.info[idx] = make_int4((int)size_d, (int)fc, (int)nchild, (int)(mask << 8));
+        t.node[idx].info = make_int4((int)size_d, (int)fc, (int)nchild, (int)(mask << 8));
       }
     }
     __syncthreads();
@@ -225,8 +224,8 @@ __global__ __launch_bounds__(256) void node64_kernel(DevTree t, const DevScalars
                                                      uint32_t* __restrict__ out) {
   const uint32_t num = sc->num_nodes;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
-    const float4 c = t.cm[i];
-    const int4 inf = t.info[i];
+    const float4 c = t.node[i].cm;
+    const int4 inf = t.node[i].info;
     uint32_t w[16];
     w[0] = __float_as_uint(c.x);
     w[1] = __float_as_uint(c.y);
@@ -270,11 +269,10 @@ __global__ __launch_bounds__(256) void import64_kernel(DevTree t, DevScalars* sc
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
     const uint4* r = (const uint4*)(in + (size_t)i * 16);
     const uint4 a = r[0], b = r[1], c = r[2], d = r[3];
-    t.cm[i] = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
+    t.node[i].cm = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
                           __uint_as_float(a.w));
-    if (i == 0) t.parent[0] = -1;
     if (b.y == 1u) {
-      t.info[i] = make_int4((int)b.x, (int)b.z, (int)b.w, 1);
+      t.node[i].info = make_int4((int)b.x, (int)b.z, (int)b.w, 1);
       continue;
     }
     const uint32_t ch[8] = {c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
@@ -286,10 +284,9 @@ __global__ __launch_bounds__(256) void import64_kernel(DevTree t, DevScalars* sc
         if (ch[k] != fc + nc) atomicOr(&sc->status, GS_ST_NODE_POOL);
         mask |= 1u << k;
         nc++;
-        if (ch[k] < num) t.parent[ch[k]] = (int32_t)i;
       }
     }
-    t.info[i] = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)nc, (int)(mask << 8));
+    t.node[i].info = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)nc, (int)(mask << 8));
   }
 }
 

@@ -5,10 +5,10 @@
 //   posm[2][N]  float4 (x, y, z, m)      ping-pong particle positions + mass
 //   velp[2][N]  float4 (vx, vy, vz, pad) ping-pong velocities + gas flag
 //   keys[2][N], vals[2][N]   u32         radix-sort ping-pong (code, index)
-//   node_cm[C]  float4 (comx, comy, comz, mass)
-//   node_info[C] int4  (size bits, first|firstChild, count|nChild, flags)
-//                      flags: bit 0 = leaf, bits 8..15 = child-octant mask
-//   node_parent[C] i32, node_arrivals[C] u32 (refresh bottom-up counters)
+//   node[C]     32-byte record: float4 (comx, comy, comz, mass) followed by
+//               int4 (size bits, first|firstChild, count|nChild, flags)
+//               flags: bit 0 = leaf, bits 8..15 = child-octant mask.  One
+//               record is one s_load_dwordx8 in the walk.
 // with C = node capacity >= 2N + 4096 (mortonTree.js:88).  Children of a BFS
 // node are contiguous, so an internal node stores (firstChild, nChild, mask)
 // instead of 8 slots; gs_download_nodes expands to the 64-byte layout.
@@ -72,11 +72,13 @@ struct DevScalars {
   uint64_t interactions;
 };
 
+struct __align__(32) NodeRec {
+  float4 cm;
+  int4 info;
+};
+
 struct DevTree {
-  float4* cm;
-  int4* info;
-  int32_t* parent;
-  uint32_t* arrivals;
+  NodeRec* node;
   uint32_t cap;
 };
 

@@ -22,7 +22,10 @@
 // built with -ffp-contract=off), so decisions match the transliteration bit
 // for bit.  EXACT evaluates f = G*m / (d2*sqrt(d2)) with IEEE sqrt and
 // division (bit-identical forces); the default path uses the hardware
-// reciprocal square root, f = G*m * rsq(d2)^3.
+// reciprocal square root, f = m * rsq(d2)^3, fused multiply-add accumulation
+// and one multiplication by G at the end (same interaction list, <= 1e-5).
+// Node records are one 32-byte scalar load; leaf members are read four at a
+// time (one 64-byte scalar load).
 //
 // After the walk (STEP): Hernquist halos, kick-drift, gas drag, pad density
 // and specific energy (nbody_bh.wgsl:144-180), then per-block FP64 partial
@@ -32,23 +35,44 @@
 
 namespace {
 
-template <bool EXACT>
-__device__ __forceinline__ void interact(float gm, float rx, float ry, float rz, float d2,
-                                         float& ax, float& ay, float& az, float& pot) {
-  if (EXACT) {
-    const float f = gm / (d2 * __builtin_sqrtf(d2));
-    ax = ax + rx * f;
-    ay = ay + ry * f;
-    az = az + rz * f;
-    pot = pot - f * d2;
-  } else {
-    const float inv = __builtin_amdgcn_rsqf(d2);
-    const float gi = gm * inv;
-    const float f = gi * (inv * inv);
-    ax = ax + rx * f;
-    ay = ay + ry * f;
-    az = az + rz * f;
-    pot = pot - gi;
+// EXACT: the reference expression f = G*m / (d2*sqrt(d2)), IEEE sqrt and '/',
+// separate multiply and add (bit-identical to the transliteration).
+__device__ __forceinline__ void interact_exact(float gm, float rx, float ry, float rz, float d2,
+                                               float& ax, float& ay, float& az, float& pot) {
+  const float f = gm / (d2 * __builtin_sqrtf(d2));
+  ax = ax + rx * f;
+  ay = ay + ry * f;
+  az = az + rz * f;
+  pot = pot - f * d2;
+}
+
+// Default path: m * rsq(d2)^3 with fused accumulation; G multiplies the sums
+// once at the end of the walk.
+__device__ __forceinline__ void interact_fast(float m, float rx, float ry, float rz, float d2,
+                                              float& ax, float& ay, float& az, float& pot) {
+  const float inv = __builtin_amdgcn_rsqf(d2);
+  const float mi = m * inv;
+  const float f = mi * (inv * inv);
+  ax = __builtin_fmaf(rx, f, ax);
+  ay = __builtin_fmaf(ry, f, ay);
+  az = __builtin_fmaf(rz, f, az);
+  pot = pot - mi;
+}
+
+template <bool EXACT, bool COUNT>
+__device__ __forceinline__ void leaf_pair(float4 q, uint32_t j, uint32_t i, bool need, float4 p,
+                                          float G, float eps2, float& ax, float& ay, float& az,
+                                          float& pot, uint32_t& cnt) {
+  if (need && j != i) {
+    const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
+    if (EXACT) {
+      const float dj2 = ((qx * qx + qy * qy) + qz * qz) + eps2;
+      interact_exact(G * q.w, qx, qy, qz, dj2, ax, ay, az, pot);
+    } else {
+      const float dj2 = __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, __builtin_fmaf(qz, qz, eps2)));
+      interact_fast(q.w, qx, qy, qz, dj2, ax, ay, az, pot);
+    }
+    if (COUNT) cnt++;
   }
 }
 
@@ -68,15 +92,19 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     const float4* __restrict__ posm, const float4* __restrict__ velp,
     float4* __restrict__ posm_out, float4* __restrict__ velp_out, float* __restrict__ energy,
     float4* __restrict__ acc_out, float* __restrict__ dens_out, uint32_t* __restrict__ inter_out,
-    const float4* __restrict__ ncm, const int4* __restrict__ ninfo,
-    const DevParams* __restrict__ prm, DevScalars* sc, double* __restrict__ red, uint32_t n) {
+    const NodeRec* __restrict__ nodes, const DevParams* __restrict__ prm, DevScalars* sc,
+    double* __restrict__ red, uint32_t n) {
   extern __shared__ uint64_t smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
   uint64_t* stm = smem + (size_t)w * GS_WALK_STACK;
   int32_t* stn = (int32_t*)(smem + (size_t)waves * GS_WALK_STACK) + (size_t)w * GS_WALK_STACK;
 
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  // Blocks keep the hardware's round-robin spread over the 8 XCDs: every XCD
+  // then walks a mix of dense and sparse regions (contiguous Morton slices
+  // per XCD were measured slower: the dense-core slices finish last).
+  const uint32_t blk = blockIdx.x;
+  const uint32_t i = blk * blockDim.x + threadIdx.x;
   const bool valid = i < n;
   const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
@@ -98,8 +126,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     sp--;
     const int32_t nd = __builtin_amdgcn_readfirstlane(stn[sp]);
     const uint64_t msk = rfl64(stm[sp]);
-    const float4 c = ncm[nd];
-    const int4 inf = ninfo[nd];
+    const NodeRec rec = nodes[nd];  // one 32-byte scalar load
+    const float4 c = rec.cm;
+    const int4 inf = rec.info;
     if (c.w == 0.0f) continue;
     const bool act = (msk >> lane) & 1ull;
     const float size = __int_as_float(inf.x);
@@ -107,26 +136,31 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane(inf.y);
     const uint32_t count = (uint32_t)__builtin_amdgcn_readfirstlane(inf.z);
     const float rx = c.x - p.x, ry = c.y - p.y, rz = c.z - p.z;
-    const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
+    const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;  // exact: drives the decision
     const float s2 = size * size;
-    if (act && leaf && i - first < count) dens = (float)count / (s2 * size);
     const bool mono = act && (s2 < theta2 * d2 || s2 < qeps2);
     if (mono) {
-      interact<EXACT>(G * c.w, rx, ry, rz, d2, ax, ay, az, pot);
+      if (EXACT)
+        interact_exact(G * c.w, rx, ry, rz, d2, ax, ay, az, pot);
+      else
+        interact_fast(c.w, rx, ry, rz, d2, ax, ay, az, pot);
       if (COUNT) cnt++;
     }
     const bool need = act && !mono;
     if (leaf) {
+      if (act && i - first < count) dens = (float)count / (s2 * size);
       if (__ballot(need)) {
-        for (uint32_t j = first; j < first + count; j++) {
-          const float4 q = posm[j];
-          if (need && j != i) {
-            const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
-            const float dj2 = ((qx * qx + qy * qy) + qz * qz) + eps2;
-            interact<EXACT>(G * q.w, qx, qy, qz, dj2, ax, ay, az, pot);
-            if (COUNT) cnt++;
-          }
+        const uint32_t last = first + count;
+        uint32_t j = first;
+        for (; j + 4 <= last; j += 4) {  // 4 members per 64-byte scalar load
+          const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+          leaf_pair<EXACT, COUNT>(q0, j, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+          leaf_pair<EXACT, COUNT>(q1, j + 1, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+          leaf_pair<EXACT, COUNT>(q2, j + 2, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+          leaf_pair<EXACT, COUNT>(q3, j + 3, i, need, p, G, eps2, ax, ay, az, pot, cnt);
         }
+        for (; j < last; j++) leaf_pair<EXACT, COUNT>(posm[j], j, i, need, p, G, eps2, ax, ay, az,
+                                                      pot, cnt);
       }
     } else {
       const uint64_t om = __ballot(need);
@@ -143,6 +177,12 @@ __global__ __launch_bounds__(1024) void walk_kernel(
         }
       }
     }
+  }
+  if (!EXACT) {
+    ax *= G;
+    ay *= G;
+    az *= G;
+    pot *= G;
   }
 
   if (COUNT) {
@@ -221,7 +261,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   if (threadIdx.x < GS_NRED) {
     double s = 0;
     for (int k = 0; k < waves; k++) s += s_red[k][threadIdx.x];
-    red[(size_t)blockIdx.x * GS_NRED + threadIdx.x] = s;
+    red[(size_t)blk * GS_NRED + threadIdx.x] = s;
   }
 }
 
@@ -248,7 +288,7 @@ hipError_t launch(gs_sim* s) {
   const int c = s->cur;
   walk_kernel<EXACT, STEP, COUNT><<<grid, wg, shm, s->stream>>>(
       s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
-      s->inter, s->tree.cm, s->tree.info, s->prm, s->sc, s->red_part, s->n);
+      s->inter, s->tree.node, s->prm, s->sc, s->red_part, s->n);
   return hipGetLastError();
 }
 
