@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact", action="store_true", help="IEEE sqrt/div force path")
+    ap.add_argument("--lib", default=None, help="path of the libgalaxy_hip.so to load")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -97,6 +98,8 @@ def main():
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.lib:
+        os.environ["GALAXY_HIP_LIB"] = args.lib
     import galaxy_hip
     from galaxy_hip import ic
 

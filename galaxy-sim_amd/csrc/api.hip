@@ -247,7 +247,7 @@ void gs_params_default(gs_params* p) {
 int gs_create(const gs_config* cfg, gs_sim** out) {
   if (!cfg || !out) return GS_E_INVALID;
   *out = nullptr;
-  if (cfg->n == 0 || cfg->n >= (1u << 30)) return GS_E_INVALID;
+  if (cfg->n == 0 || cfg->n > GS_MAX_N) return GS_E_INVALID;
   uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
   if (wg < 64 || wg > 1024 || (wg & 63)) return GS_E_INVALID;
   if (cfg->radix_bits != 0 && cfg->radix_bits != GS_RADIX_BITS) return GS_E_INVALID;
@@ -398,16 +398,25 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
          "node count outside [1, pool]");
     need(num_levels <= GS_MAX_LEVELS && (num_levels == 0 || level_ranges), GS_E_INVALID,
          "bad level ranges");
-    // host validation: every index the device will follow stays in bounds
+    // host validation: every index the device will follow stays in bounds;
+    // the walk's (skip, depth) links follow from the BFS numbering
+    std::vector<int2> links(num_nodes, make_int2(-1, 0));
     for (uint64_t i = 0; i < num_nodes; i++) {
       const uint32_t* w = nodes + i * 16;
       if (w[5] == 1u) {
         need(w[6] <= s->n && w[7] <= s->n - w[6], GS_E_INVALID, "leaf range outside [0, N)");
       } else {
+        int32_t prev = -1;
         for (int k = 0; k < 8; k++) {
           const int32_t c = (int32_t)w[8 + k];
           need(c < 0 || ((uint64_t)c > i && (uint64_t)c < num_nodes), GS_E_INVALID,
                "child index must follow its parent and lie inside the tree");
+          if (c >= 0) {
+            need(links[c].y == 0, GS_E_INVALID, "node has two parents");
+            links[c] = make_int2(prev >= 0 ? prev : links[i].x, links[i].y + 1);
+            need(links[c].y <= GS_MAX_LEVEL, GS_E_INVALID, "tree deeper than 10 levels");
+            prev = c;
+          }
         }
       }
     }
@@ -425,7 +434,14 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
     }
     h.status = 0;
     check(hipMemcpyAsync(s->sc, &h, sizeof h, hipMemcpyHostToDevice, s->stream), "scalars");
-    check(gs_launch_import64(s, s->node64, (uint32_t)num_nodes), "import");
+    int2* dlinks = nullptr;
+    dalloc(&dlinks, num_nodes, "links");
+    hipError_t e = hipMemcpyAsync(dlinks, links.data(), sizeof(int2) * num_nodes,
+                                  hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = gs_launch_import64(s, s->node64, dlinks, (uint32_t)num_nodes);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dlinks);
+    check(e, "import");
     check_status(s);  // non-contiguous children are rejected here
     s->have_tree = true;
     s->have_force = false;
@@ -709,7 +725,7 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     h.status = 0;
     for (int k = 0; k < 32; k++) h.tickets[k] = 0;
     check(hipMemcpyAsync(s->sc, &h, sizeof h, hipMemcpyHostToDevice, s->stream), "scalars");
-    int4 root = make_int4(0, 0, (int)n, 0);
+    int4 root = make_int4(0, 0, (int)gs_meta_open(0, n), -1);
     float f0 = h.level_size[0];
     memcpy(&root.x, &f0, 4);
     check(hipMemcpyAsync(&s->tree.node[0].info, &root, sizeof root, hipMemcpyHostToDevice,

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
       if (local < count) {
         const uint32_t idx = start + local;
         const int4 inf = t.node[idx].info;
-        const uint32_t lo = (uint32_t)inf.y, cnt = (uint32_t)inf.z;
+        const uint32_t lo = (uint32_t)inf.y, cnt = gs_meta_count((uint32_t)inf.z);
         const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
         uint32_t b = lo;
         if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
@@ -179,8 +179,11 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
 
     if (valid) {
       const int4 inf = t.node[idx].info;
+      const int32_t skip = inf.w;
       if (leaf) {
-        t.node[idx].info = make_int4((int)size_d, inf.y, inf.z, 1);
+        t.node[idx].info = make_int4((int)size_d, inf.y,
+                                     (int)gs_meta_leaf((uint32_t)d, gs_meta_count((uint32_t)inf.z)),
+                                     skip);
       } else {
         const uint32_t fc = next_start + (uint32_t)s_prefix + excl_local;
         const uint32_t hi = s_b[threadIdx.x][8];
@@ -191,12 +194,16 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
             const uint32_t b0 = s_b[threadIdx.x][k];
             const uint32_t b1 = k < 7 ? s_b[threadIdx.x][k + 1] : hi;
             if (c < t.cap) {
-              t.node[c].info = make_int4((int)size_c, (int)b0, (int)(b1 - b0), 0);
+              // pre-order with octant 7 first: after child c comes c - 1
+              t.node[c].info = make_int4((int)size_c, (int)b0,
+                                         (int)gs_meta_open((uint32_t)d + 1, b1 - b0),
+                                         c > fc ? (int32_t)c - 1 : skip);
             }
             c++;
           }
         }
-        t.node[idx].info = make_int4((int)size_d, (int)fc, (int)nchild, (int)(mask << 8));
+        t.node[idx].info = make_int4((int)size_d, (int)fc,
+                                     (int)gs_meta_internal((uint32_t)d, mask, nchild), skip);
       }
     }
     __syncthreads();
@@ -232,11 +239,12 @@ __global__ __launch_bounds__(256) void node64_kernel(DevTree t, const DevScalars
     w[2] = __float_as_uint(c.z);
     w[3] = __float_as_uint(c.w);
     w[4] = (uint32_t)inf.x;
-    const bool leaf = inf.w & 1;
+    const uint32_t meta = (uint32_t)inf.z;
+    const bool leaf = gs_meta_is_leaf(meta);
     w[5] = leaf ? 1u : 0u;
     w[6] = leaf ? (uint32_t)inf.y : 0u;
-    w[7] = leaf ? (uint32_t)inf.z : 0u;
-    const uint32_t mask = leaf ? 0u : ((uint32_t)inf.w >> 8) & 0xffu;
+    w[7] = leaf ? gs_meta_count(meta) : 0u;
+    const uint32_t mask = leaf ? 0u : gs_meta_mask(meta);
     uint32_t c0 = (uint32_t)inf.y;
     for (int k = 0; k < 8; k++) {
       if (mask & (1u << k)) {
@@ -265,14 +273,17 @@ hipError_t gs_launch_node64(gs_sim* s) {
 namespace {
 __global__ __launch_bounds__(256) void import64_kernel(DevTree t, DevScalars* sc,
                                                        const uint32_t* __restrict__ in,
+                                                       const int2* __restrict__ links,
                                                        uint32_t num) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
     const uint4* r = (const uint4*)(in + (size_t)i * 16);
     const uint4 a = r[0], b = r[1], c = r[2], d = r[3];
     t.node[i].cm = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
-                          __uint_as_float(a.w));
+                               __uint_as_float(a.w));
+    const int32_t skip = links[i].x;
+    const uint32_t depth = (uint32_t)links[i].y;
     if (b.y == 1u) {
-      t.node[i].info = make_int4((int)b.x, (int)b.z, (int)b.w, 1);
+      t.node[i].info = make_int4((int)b.x, (int)b.z, (int)gs_meta_leaf(depth, b.w), skip);
       continue;
     }
     const uint32_t ch[8] = {c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
@@ -286,7 +297,8 @@ __global__ __launch_bounds__(256) void import64_kernel(DevTree t, DevScalars* sc
         nc++;
       }
     }
-    t.node[i].info = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)nc, (int)(mask << 8));
+    t.node[i].info = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)gs_meta_internal(depth, mask, nc),
+                               skip);
   }
 }
 
@@ -303,11 +315,11 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict_
 }
 }  // namespace
 
-hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, uint32_t num) {
+hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, const int2* links, uint32_t num) {
   uint32_t grid = (num + 255) / 256;
   if (grid > 4096) grid = 4096;
   if (grid < 1) grid = 1;
-  import64_kernel<<<grid, 256, 0, s->stream>>>(s->tree, s->sc, words, num);
+  import64_kernel<<<grid, 256, 0, s->stream>>>(s->tree, s->sc, words, links, num);
   return hipGetLastError();
 }
 

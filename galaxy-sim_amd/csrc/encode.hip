@@ -166,7 +166,8 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   }
   sc->num_nodes = 1;
   sc->num_levels = 0;
-  t.node[0].info = make_int4((int)__float_as_uint(sc->level_size[0]), 0, (int)n, 0);
+  t.node[0].info = make_int4((int)__float_as_uint(sc->level_size[0]), 0,
+                             (int)gs_meta_open(0, n), -1);
 }
 
 __device__ __forceinline__ uint32_t quant(float p, uint64_t mn, uint64_t s) {

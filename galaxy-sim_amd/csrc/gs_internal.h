@@ -5,10 +5,15 @@
 //   posm[2][N]  float4 (x, y, z, m)      ping-pong particle positions + mass
 //   velp[2][N]  float4 (vx, vy, vz, pad) ping-pong velocities + gas flag
 //   keys[2][N], vals[2][N]   u32         radix-sort ping-pong (code, index)
-//   node[C]     32-byte record: float4 (comx, comy, comz, mass) followed by
-//               int4 (size bits, first|firstChild, count|nChild, flags)
-//               flags: bit 0 = leaf, bits 8..15 = child-octant mask.  One
-//               record is one s_load_dwordx8 in the walk.
+//   node[C]     32-byte record (one s_load_dwordx8 in the walk):
+//               float4 cm   (comx, comy, comz, mass)
+//               int4   info (size bits, first | firstChild, meta, skip)
+//               meta: bit 31 leaf, bits 27..30 depth, then
+//                     leaf: bits 0..26 particle count
+//                     internal: bits 0..3 child count, bits 8..15 octant mask
+//               skip: the node that follows this node's subtree in the
+//                     walk's pre-order (previous sibling, else the parent's
+//                     skip; -1 ends the walk)
 // with C = node capacity >= 2N + 4096 (mortonTree.js:88).  Children of a BFS
 // node are contiguous, so an internal node stores (firstChild, nChild, mask)
 // instead of 8 slots; gs_download_nodes expands to the 64-byte layout.
@@ -23,7 +28,6 @@
 
 #define GS_MAX_LEVEL 10   // mortonTree.js:26
 #define GS_MAX_LEVELS 16  // level slots (NBodyGPU.js:171-174)
-#define GS_WALK_STACK 80  // per-wavefront stack; the walk provably needs <= 71
 
 // status bits raised by kernels, checked by the host at sync points
 #define GS_ST_NODE_POOL 0x1u
@@ -76,6 +80,26 @@ struct __align__(32) NodeRec {
   float4 cm;
   int4 info;
 };
+
+// node meta word
+#define GS_META_LEAF 0x80000000u
+#define GS_MAX_N ((1u << 27) - 1)  // a leaf count must fit the 27-bit field
+__host__ __device__ __forceinline__ uint32_t gs_meta_leaf(uint32_t depth, uint32_t count) {
+  return GS_META_LEAF | (depth << 27) | count;
+}
+__host__ __device__ __forceinline__ uint32_t gs_meta_internal(uint32_t depth, uint32_t mask,
+                                                             uint32_t nc) {
+  return (depth << 27) | (mask << 8) | nc;
+}
+// a node emitted by its parent, not yet classified: depth + particle count
+__host__ __device__ __forceinline__ uint32_t gs_meta_open(uint32_t depth, uint32_t count) {
+  return (depth << 27) | count;
+}
+__host__ __device__ __forceinline__ bool gs_meta_is_leaf(uint32_t m) { return (m >> 31) != 0; }
+__host__ __device__ __forceinline__ uint32_t gs_meta_depth(uint32_t m) { return (m >> 27) & 0xfu; }
+__host__ __device__ __forceinline__ uint32_t gs_meta_count(uint32_t m) { return m & 0x7ffffffu; }
+__host__ __device__ __forceinline__ uint32_t gs_meta_nc(uint32_t m) { return m & 0xfu; }
+__host__ __device__ __forceinline__ uint32_t gs_meta_mask(uint32_t m) { return (m >> 8) & 0xffu; }
 
 struct DevTree {
   NodeRec* node;
@@ -142,7 +166,7 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm);
 hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count, bool reduce);
 hipError_t gs_launch_reduce_final(gs_sim* s, double* out_dev);
 hipError_t gs_launch_node64(gs_sim* s);
-hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, uint32_t num);
+hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, const int2* links, uint32_t num);
 hipError_t gs_launch_gather(gs_sim* s, const uint32_t* perm);
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t gs_launch_pack_soa(gs_sim* s, const float* aos, int which);

@@ -36,9 +36,9 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
   for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < num;
        idx += gridDim.x * blockDim.x) {
     const int4 inf = nodes[idx].info;
-    if ((inf.w & 1) == 0) continue;
+    if (!gs_meta_is_leaf((uint32_t)inf.z)) continue;
     double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    const uint32_t first = (uint32_t)inf.y, last = first + (uint32_t)inf.z;
+    const uint32_t first = (uint32_t)inf.y, last = first + gs_meta_count((uint32_t)inf.z);
     for (uint32_t j = first; j < last; j++) {
       const float4 p = posm[j];
       const double pm = (double)p.w;
@@ -60,9 +60,9 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
        t += gridDim.x * blockDim.x) {
     const uint32_t idx = start + t;
     const int4 inf = nodes[idx].info;
-    if (inf.w & 1) continue;
+    if (gs_meta_is_leaf((uint32_t)inf.z)) continue;
     double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    const uint32_t fc = (uint32_t)inf.y, nc = (uint32_t)inf.z;
+    const uint32_t fc = (uint32_t)inf.y, nc = gs_meta_nc((uint32_t)inf.z);
     for (uint32_t c = fc; c < fc + nc; c++) {
       const float4 q = nodes[c].cm;
       const double qm = (double)q.w;

@@ -1,21 +1,22 @@
 // Barnes-Hut walk + integrate — the `main` kernel of nbody_bh.wgsl:64-181.
 //
-// Wavefront-coherent traversal with per-lane exactness.  The 64 lanes of a
-// wavefront own 64 consecutive Morton-sorted particles and share ONE stack in
-// LDS whose entries are (node, 64-bit lane mask).  Popping an entry, the
-// lanes in its mask evaluate the reference's own per-particle decision for
-// that node (nbody_bh.wgsl:97-141): monopole if s^2 < theta^2 d^2 or
-// s^2 < eps^2/4, else leaf direct sum (skipping j == i), else open.  The
-// children of an opened node are pushed with the mask of the lanes that
-// opened it, octant 0 first, so every lane visits exactly the nodes its own
-// per-thread stack would visit, in the same order: its interaction list and
-// its summation order equal the reference walk's.  The node index is
-// wave-uniform, so node and leaf-member loads are scalar (SMEM) broadcast
-// loads instead of 64 divergent vector loads, and divergence costs only the
-// lanes masked off.
-//
-// The shared stack never holds more than 7 * 9 + 8 = 71 entries (the DFS of
-// the union tree keeps <= 7 unvisited siblings per open level, depth <= 10).
+// Wavefront-coherent, stackless traversal with per-lane exactness.  The 64
+// lanes of a wavefront own 64 consecutive Morton-sorted particles and walk
+// ONE node sequence together: the pre-order of the union of their trees, in
+// the order the reference's per-thread stack pops (children pushed octant 0
+// first, so octant 7 is visited first).  Each node record carries its depth
+// and a skip link (the node that follows its subtree), and each lane keeps a
+// bit per depth saying whether it opened the ancestor at that depth.  At a
+// node, the active lanes evaluate the reference's own per-particle decision
+// (nbody_bh.wgsl:97-141): monopole if s^2 < theta^2 d^2 or s^2 < eps^2/4,
+// else leaf direct sum (skipping j == i), else open.  If any lane opens, the
+// wave descends to the last child with those lanes' bit set; otherwise it
+// follows the skip link.  Every lane therefore visits exactly the nodes its
+// own stack would pop, in the same order: its interaction list and its
+// summation order equal the reference walk's.  The node index is
+// wave-uniform, so node records and leaf members are scalar (SMEM) broadcast
+// loads, there is no stack in memory, and divergence costs only the lanes
+// masked off.
 //
 // Arithmetic: the opening test uses the exact reference expression order
 // d2 = ((rx*rx + ry*ry) + rz*rz) + eps2 with no contraction (the library is
@@ -63,17 +64,28 @@ template <bool EXACT, bool COUNT>
 __device__ __forceinline__ void leaf_pair(float4 q, uint32_t j, uint32_t i, bool need, float4 p,
                                           float G, float eps2, float& ax, float& ay, float& az,
                                           float& pot, uint32_t& cnt) {
-  if (need && j != i) {
-    const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
-    if (EXACT) {
+  const bool take = need && j != i;
+  const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
+  if (EXACT) {
+    if (take) {
       const float dj2 = ((qx * qx + qy * qy) + qz * qz) + eps2;
       interact_exact(G * q.w, qx, qy, qz, dj2, ax, ay, az, pot);
-    } else {
-      const float dj2 = __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, __builtin_fmaf(qz, qz, eps2)));
-      interact_fast(q.w, qx, qy, qz, dj2, ax, ay, az, pot);
     }
-    if (COUNT) cnt++;
+  } else {
+    // branchless: masked-off lanes cost the same issue slot, so they compute
+    // with zero mass instead of branching (keeps the member loads batched)
+    const float dj2 = __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, __builtin_fmaf(qz, qz, eps2)));
+    interact_fast(take ? q.w : 0.0f, qx, qy, qz, dj2, ax, ay, az, pot);
   }
+  if (COUNT && take) cnt++;
+}
+
+// default-path leaf member with a 0/1 lane weight (no index test)
+__device__ __forceinline__ void pair_weighted(float4 q, float w, float4 p, float eps2, float& ax,
+                                              float& ay, float& az, float& pot) {
+  const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
+  const float dj2 = __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, __builtin_fmaf(qz, qz, eps2)));
+  interact_fast(q.w * w, qx, qy, qz, dj2, ax, ay, az, pot);
 }
 
 __device__ __forceinline__ uint64_t rfl64(uint64_t v) {
@@ -94,11 +106,8 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     float4* __restrict__ acc_out, float* __restrict__ dens_out, uint32_t* __restrict__ inter_out,
     const NodeRec* __restrict__ nodes, const DevParams* __restrict__ prm, DevScalars* sc,
     double* __restrict__ red, uint32_t n) {
-  extern __shared__ uint64_t smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
-  uint64_t* stm = smem + (size_t)w * GS_WALK_STACK;
-  int32_t* stn = (int32_t*)(smem + (size_t)waves * GS_WALK_STACK) + (size_t)w * GS_WALK_STACK;
 
   // Blocks keep the hardware's round-robin spread over the 8 XCDs: every XCD
   // then walks a mix of dense and sparse regions (contiguous Morton slices
@@ -113,31 +122,24 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
   uint32_t cnt = 0;
 
-  const uint64_t all = __ballot(valid);
-  int sp = 0;
-  if (all) {
-    if (lane == 0) {
-      stn[0] = 0;
-      stm[0] = all;
-    }
-    sp = 1;
-  }
-  while (sp > 0) {
-    sp--;
-    const int32_t nd = __builtin_amdgcn_readfirstlane(stn[sp]);
-    const uint64_t msk = rfl64(stm[sp]);
+  uint32_t open_bits = valid ? 1u : 0u;  // bit d: this lane is active at depth d
+  int32_t nd = __ballot(valid) ? 0 : -1;
+  while (nd >= 0) {
     const NodeRec rec = nodes[nd];  // one 32-byte scalar load
     const float4 c = rec.cm;
     const int4 inf = rec.info;
-    if (c.w == 0.0f) continue;
-    const bool act = (msk >> lane) & 1ull;
+    const uint32_t meta = (uint32_t)inf.z;
+    const uint32_t depth = gs_meta_depth(meta);
+    const int32_t skip = inf.w;
+    // a massless node is skipped by every lane (nbody_bh.wgsl:86)
+    const bool act = ((open_bits >> depth) & 1u) && c.w != 0.0f;
     const float size = __int_as_float(inf.x);
-    const bool leaf = (inf.w & 1) != 0;
-    const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane(inf.y);
-    const uint32_t count = (uint32_t)__builtin_amdgcn_readfirstlane(inf.z);
+    const float s2 = size * size;
+    const bool leaf = gs_meta_is_leaf(meta);
+    const uint32_t first = (uint32_t)inf.y;
+    const uint32_t count = leaf ? gs_meta_count(meta) : gs_meta_nc(meta);
     const float rx = c.x - p.x, ry = c.y - p.y, rz = c.z - p.z;
     const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;  // exact: drives the decision
-    const float s2 = size * size;
     const bool mono = act && (s2 < theta2 * d2 || s2 < qeps2);
     if (mono) {
       if (EXACT)
@@ -148,33 +150,47 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     }
     const bool need = act && !mono;
     if (leaf) {
-      if (act && i - first < count) dens = (float)count / (s2 * size);
+      const bool own = act && i - first < count;
+      if (own) dens = (float)count / (s2 * size);
       if (__ballot(need)) {
         const uint32_t last = first + count;
         uint32_t j = first;
-        for (; j + 4 <= last; j += 4) {  // 4 members per 64-byte scalar load
-          const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
-          leaf_pair<EXACT, COUNT>(q0, j, i, need, p, G, eps2, ax, ay, az, pot, cnt);
-          leaf_pair<EXACT, COUNT>(q1, j + 1, i, need, p, G, eps2, ax, ay, az, pot, cnt);
-          leaf_pair<EXACT, COUNT>(q2, j + 2, i, need, p, G, eps2, ax, ay, az, pot, cnt);
-          leaf_pair<EXACT, COUNT>(q3, j + 3, i, need, p, G, eps2, ax, ay, az, pot, cnt);
-        }
-        for (; j < last; j++) leaf_pair<EXACT, COUNT>(posm[j], j, i, need, p, G, eps2, ax, ay, az,
-                                                      pot, cnt);
-      }
-    } else {
-      const uint64_t om = __ballot(need);
-      if (om) {
-        const int nc = (int)count;  // internal: child count
-        if (sp + nc > GS_WALK_STACK) {
-          if (lane == 0) atomicOr(&sc->status, GS_ST_STACK);
-        } else {
-          if (lane < nc) {
-            stn[sp + lane] = (int32_t)first + lane;  // internal: first child
-            stm[sp + lane] = om;
+        if (!EXACT && !COUNT && eps2 > 0.0f) {
+          // eps > 0: the j == i term adds an exact zero to the acceleration
+          // (r = 0, f finite), so members run without a per-pair index test
+          // and the self potential is taken back once per leaf
+          const float wneed = need ? 1.0f : 0.0f;
+          for (; j + 4 <= last; j += 4) {
+            const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+            pair_weighted(q0, wneed, p, eps2, ax, ay, az, pot);
+            pair_weighted(q1, wneed, p, eps2, ax, ay, az, pot);
+            pair_weighted(q2, wneed, p, eps2, ax, ay, az, pot);
+            pair_weighted(q3, wneed, p, eps2, ax, ay, az, pot);
           }
-          sp += nc;
+          for (; j < last; j++) pair_weighted(posm[j], wneed, p, eps2, ax, ay, az, pot);
+          if (need && own) pot = pot + p.w * __builtin_amdgcn_rsqf(eps2);
+        } else {
+          for (; j + 4 <= last; j += 4) {  // 4 members per batch of scalar loads
+            const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+            leaf_pair<EXACT, COUNT>(q0, j, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+            leaf_pair<EXACT, COUNT>(q1, j + 1, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+            leaf_pair<EXACT, COUNT>(q2, j + 2, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+            leaf_pair<EXACT, COUNT>(q3, j + 3, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+          }
+          for (; j < last; j++)
+            leaf_pair<EXACT, COUNT>(posm[j], j, i, need, p, G, eps2, ax, ay, az, pot, cnt);
         }
+      }
+      nd = skip;
+    } else {
+      // descend to the last child (visited first) with the opening lanes
+      // active at the children's depth, else continue after the subtree
+      if (__ballot(need)) {
+        const uint32_t bit = 2u << depth;
+        open_bits = need ? (open_bits | bit) : (open_bits & ~bit);
+        nd = (int32_t)(first + count - 1);  // internal: firstChild + nChild - 1
+      } else {
+        nd = skip;
       }
     }
   }
@@ -284,9 +300,8 @@ template <bool EXACT, bool STEP, bool COUNT>
 hipError_t launch(gs_sim* s) {
   const uint32_t wg = s->walk_wg;
   const uint32_t grid = (s->n + wg - 1) / wg;
-  const size_t shm = (size_t)(wg / 64) * GS_WALK_STACK * (sizeof(uint64_t) + sizeof(int32_t));
   const int c = s->cur;
-  walk_kernel<EXACT, STEP, COUNT><<<grid, wg, shm, s->stream>>>(
+  walk_kernel<EXACT, STEP, COUNT><<<grid, wg, 0, s->stream>>>(
       s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
       s->inter, s->tree.node, s->prm, s->sc, s->red_part, s->n);
   return hipGetLastError();

@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "lib", "libgalaxy_hip.so")
+LIB_PATH = os.environ.get("GALAXY_HIP_LIB") or os.path.join(_PKG, "lib", "libgalaxy_hip.so")
 
 GS_OK = 0
 GS_E_INVALID = -1

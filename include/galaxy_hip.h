@@ -56,7 +56,7 @@ extern "C" {
 typedef struct gs_sim gs_sim;
 
 typedef struct gs_config {
-  uint32_t n;         /* particles, N (NBodyGPU.js:17); 1 .. 2^30 */
+  uint32_t n;         /* particles, N (NBodyGPU.js:17); 1 .. 2^27 - 1 */
   uint32_t bucket;    /* leaf capacity BUCKET (mortonTree.js:25); 0 -> 16 */
   uint64_t max_nodes; /* leaf-rule pool size maxNodes (mortonTree.js:88,127);
                          0 -> 2n + 4096.  The device pool always holds at

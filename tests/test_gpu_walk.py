@@ -67,6 +67,8 @@ def test_fast_force_within_tolerance():
     err = rel_err(acc[:, :3], racc[:, :3])
     banner(f"fast path: max rel err {err.max():.3e}, p99 {np.quantile(err, 0.99):.3e}")
     assert err.max() <= FAST_TOL
+    perr = np.abs(acc[:, 3].astype(np.float64) - racc[:, 3]) / np.abs(racc[:, 3].astype(np.float64))
+    assert perr.max() <= FAST_TOL  # pair potential (feeds the energy diagnostics)
     assert np.array_equal(dens, rdens)
 
 
