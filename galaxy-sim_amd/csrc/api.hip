@@ -1,7 +1,8 @@
-// C ABI of libgalaxy_hip.so (include/galaxy_hip.h): handle lifetime, uploads,
-// stage sequencing on the handle's stream, downloads, and the mortonTree.js
-// functions on host arrays.  No exception crosses the ABI: every entry point
-// catches, records the message on the handle and returns a GS_E* status.
+// C ABI of libgalaxy_hip.so (include/galaxy_hip.h): pool layout, uploads,
+// stage sequencing on the handle's stream (direct launches or a replayed
+// step graph), downloads, and the mortonTree.js functions on host arrays.
+// No exception crosses the ABI: every entry point catches, records the
+// message on the handle and returns a GS_E* status.
 #include <math.h>
 #include <string.h>
 
@@ -11,7 +12,7 @@
 
 #include "gs_internal.h"
 
-#define GS_VERSION "galaxy_hip 0.1 (gfx950)"
+#define GS_VERSION "galaxy_hip 0.2 (gfx950)"
 
 namespace {
 
@@ -58,7 +59,14 @@ void need(bool ok, int code, const char* msg) {
   if (!ok) throw Fail{code, msg};
 }
 
-// ---- stage timing ---------------------------------------------------------
+HostSlot& slot_of(gs_sim* s, uint32_t slot) {
+  need(slot < s->nslots, GS_E_INVALID, "slot index out of range");
+  return s->hs[slot];
+}
+
+void sync(gs_sim* s) { check(hipStreamSynchronize(s->stream), "hipStreamSynchronize"); }
+
+// ---- stage timing (direct launches only) -----------------------------------
 struct Stage {
   gs_sim* s;
   int id;
@@ -82,7 +90,7 @@ struct Stage {
 
 void collect_timers(gs_sim* s) {
   if (s->timers.empty()) return;
-  check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  sync(s);
   for (auto& t : s->timers) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
@@ -97,16 +105,21 @@ void collect_timers(gs_sim* s) {
 
 void check_status(gs_sim* s) {
   uint32_t st = 0;
-  check(hipMemcpyAsync(&st, &s->sc->status, sizeof st, hipMemcpyDeviceToHost, s->stream),
+  check(hipMemcpyAsync(&st, &s->batch->status, sizeof st, hipMemcpyDeviceToHost, s->stream),
         "read status");
-  check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  sync(s);
   if (st & GS_ST_NODE_POOL)
     throw Fail{GS_E_NODE_POOL, "octree needs more nodes than the pool holds (raise max_nodes)"};
+  if (st & GS_ST_LAYOUT)
+    throw Fail{GS_E_INVALID, "uploaded tree: a node's children are not contiguous"};
   if (st & GS_ST_STACK) throw Fail{GS_E_STACK, "walk stack overflow"};
   if (st & GS_ST_SPIN) throw Fail{GS_E_DEVICE, "a look-back wait timed out (scan state corrupt)"};
 }
 
 void free_all(gs_sim* s) {
+  for (int p = 0; p < 2; p++)
+    for (int q = 0; q < 2; q++)
+      if (s->graph[p][q]) (void)hipGraphExecDestroy(s->graph[p][q]);
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -120,15 +133,21 @@ void free_all(gs_sim* s) {
   f(s->codes_raw);
   f(s->sort_status);
   f(s->bbox_part);
-  f(s->tree.node);
+  f(s->nodes);
   f(s->energy);
   f(s->acc);
   f(s->dens);
   f(s->inter);
   f(s->red_part);
+  f(s->red_out);
   f(s->node64);
+  f(s->slots);
   f(s->sc);
   f(s->prm);
+  f(s->batch);
+  f(s->chunk_tab);
+  f(s->tile_tab);
+  f(s->wblock_slot);
   for (auto& t : s->timers) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -164,8 +183,21 @@ void pack_params(const gs_params* p, uint32_t n, DevParams* d) {
   d->drag_damp = expf(-d->gas_drag * d->dt);
 }
 
+void push_slot(gs_sim* s, uint32_t slot) {
+  check(hipMemcpyAsync(&s->slots[slot], &s->hs[slot].d, sizeof(DevSlot), hipMemcpyHostToDevice,
+                       s->stream),
+        "slot descriptor");
+}
+
+void push_params(gs_sim* s, uint32_t slot) {
+  s->host_prm[slot].n = s->hs[slot].d.n;
+  check(hipMemcpyAsync(&s->prm[slot], &s->host_prm[slot], sizeof(DevParams),
+                       hipMemcpyHostToDevice, s->stream),
+        "params");
+}
+
 // ---- stage sequences --------------------------------------------------------
-void do_sort(gs_sim* s, bool keep_raw) {
+void launch_sort(gs_sim* s, bool keep_raw) {
   {
     Stage t(s, GS_T_ENCODE);
     check(gs_launch_bbox(s, s->posm[s->cur]), "bbox");
@@ -181,28 +213,253 @@ void do_sort(gs_sim* s, bool keep_raw) {
     Stage t(s, GS_T_BUILD);
     check(gs_launch_build(s), "build");
   }
-  s->have_tree = true;
-  s->have_force = false;
 }
 
-void do_refresh(gs_sim* s) {
+void launch_refresh(gs_sim* s) {
   Stage t(s, GS_T_REFRESH);
   check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
 }
 
-void do_walk_step(gs_sim* s) {
+void launch_walk_step(gs_sim* s) {
   {
     Stage t(s, GS_T_WALK);
-    check(gs_launch_walk(s, true, false, true), "walk");
+    check(gs_launch_walk(s, true, false), "walk");
   }
   s->cur ^= 1;
-  s->have_step = true;
-  s->have_force = false;
+}
+
+void mark_sorted(gs_sim* s) {
+  for (auto& h : s->hs) {
+    h.have_tree = h.d.n > 0;
+    h.have_force = false;
+  }
+}
+
+void mark_stepped(gs_sim* s) {
+  for (auto& h : s->hs) {
+    h.have_step = h.d.n > 0;
+    h.have_force = false;
+  }
+}
+
+bool use_graph(const gs_sim* s) { return !(s->flags & (GS_FLAG_NO_GRAPH | GS_FLAG_TIMING)); }
+
+// One step (optional rebuild + refresh + walk) as a graph for the current
+// ping-pong parity; the launch sequence depends only on static sizes, so one
+// capture serves every later step and every change of slot occupancy.
+hipGraphExec_t step_graph(gs_sim* s, bool sort) {
+  hipGraphExec_t& slot = s->graph[s->cur][sort ? 1 : 0];
+  if (slot) return slot;
+  const int cur0 = s->cur;
+  hipGraph_t g = nullptr;
+  check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    if (sort) launch_sort(s, false);
+    launch_refresh(s);
+    launch_walk_step(s);
+  } catch (...) {
+    hipGraph_t dead = nullptr;
+    (void)hipStreamEndCapture(s->stream, &dead);
+    if (dead) (void)hipGraphDestroy(dead);
+    s->cur = cur0;
+    throw;
+  }
+  check(hipStreamEndCapture(s->stream, &g), "end capture");
+  s->cur = cur0;
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  check(e, "instantiate step graph");
+  slot = ex;
+  return ex;
+}
+
+void run_step(gs_sim* s, bool sort) {
+  if (use_graph(s)) {
+    hipGraphExec_t ex = step_graph(s, sort);
+    check(hipGraphLaunch(ex, s->stream), "graph launch");
+    s->cur ^= sort ? 0 : 1;  // the sort's gather flips once, the walk once
+  } else {
+    Stage t(s, GS_T_STEP);
+    if (sort) launch_sort(s, false);
+    launch_refresh(s);
+    launch_walk_step(s);
+  }
+  if (sort) mark_sorted(s);
+  mark_stepped(s);
+}
+
+uint32_t align_up(uint64_t v, uint32_t a) { return (uint32_t)((v + a - 1) / a * a); }
+
+// Allocates a pool of nslots slots (caps[k] particles, buckets[k] leaf size,
+// a node pool of max(2 caps[k] + 4096, max_nodes) nodes).
+gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots, uint32_t flags,
+                  int device, uint32_t wg, uint64_t max_nodes) {
+  gs_sim* s = new gs_sim();
+  s->flags = flags;
+  s->walk_wg = wg;
+  s->nslots = nslots;
+  s->hs.resize(nslots);
+  s->host_prm.resize(nslots);
+  uint64_t poff = 0, noff = 0;
+  uint32_t chunk = 0, tile = 0, bt = 0, wb = 0;
+  std::vector<uint2> chunk_tab;
+  std::vector<uint32_t> tile_tab, wblock_slot;
+  for (uint32_t k = 0; k < nslots; k++) {
+    DevSlot& d = s->hs[k].d;
+    memset(&d, 0, sizeof d);
+    const uint32_t cap = caps[k];
+    d.n = 0;
+    d.cap = cap;
+    d.poff = (uint32_t)poff;
+    d.ncap = (uint32_t)std::max<uint64_t>(2ull * cap + 4096, max_nodes);  // mortonTree.js:88
+    d.noff = (uint32_t)noff;
+    d.bucket = buckets && buckets[k] ? buckets[k] : 16;
+    d.max_nodes = d.ncap;
+    d.nchunks = (cap + GS_CHUNK - 1) / GS_CHUNK;
+    d.chunk0 = chunk;
+    d.tile0 = tile;
+    for (uint32_t c = 0; c < d.nchunks; c++) {
+      chunk_tab.push_back(make_uint2(k, c * GS_CHUNK));
+      tile_tab.push_back((k << 16) | c);
+    }
+    chunk += d.nchunks;
+    tile += d.nchunks;
+    d.btoff = bt;
+    bt += (d.ncap + GS_BUILD_TILE - 1) / GS_BUILD_TILE;
+    const uint32_t span = align_up(cap, GS_SLOT_ALIGN);
+    d.wblock0 = wb;
+    d.nwblocks = span / wg;
+    for (uint32_t b = 0; b < d.nwblocks; b++) wblock_slot.push_back(k);
+    wb += d.nwblocks;
+    poff += span;
+    noff += d.ncap;
+    s->max_ncap = std::max(s->max_ncap, d.ncap);
+  }
+  s->ptot = poff;
+  s->ntot = noff;
+  s->chunks = chunk;
+  s->sort_tiles = tile;
+  s->build_words = bt;
+  s->walk_blocks = wb;
+  int rc = guarded(s, [&] {
+    need(s->ptot < (1ull << 32) && s->ntot < (1ull << 32), GS_E_INVALID,
+         "pool larger than 2^32 particles or nodes");
+    int ndev = 0;
+    check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    need(ndev > 0, GS_E_DEVICE, "no HIP device");
+    int dev = device;
+    if (dev < 0) check(hipGetDevice(&dev), "hipGetDevice");
+    need(dev < ndev, GS_E_INVALID, "device ordinal out of range");
+    s->device = dev;
+    check(hipSetDevice(dev), "hipSetDevice");
+    hipDeviceProp_t prop;
+    check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    need(strncmp(prop.gcnArchName, "gfx950", 6) == 0, GS_E_DEVICE,
+         "libgalaxy_hip is built for gfx950 (MI355X) only");
+    check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const size_t P = s->ptot;
+    for (int k = 0; k < 2; k++) {
+      dalloc(&s->posm[k], P, "posm");
+      dalloc(&s->velp[k], P, "velp");
+      dalloc(&s->keys[k], P, "keys");
+      dalloc(&s->vals[k], P, "vals");
+      dalloc(&s->build_status[k], s->build_words, "build status");
+    }
+    dalloc(&s->codes_raw, P, "codes");
+    dalloc(&s->sort_status, (size_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX, "sort status");
+    dalloc(&s->bbox_part, (size_t)s->chunks * 8, "bbox");
+    dalloc(&s->nodes, s->ntot, "nodes");
+    dalloc(&s->energy, P, "energy");
+    dalloc(&s->acc, P, "acc");
+    dalloc(&s->dens, P, "density");
+    dalloc(&s->inter, P, "interactions");
+    dalloc(&s->red_part, (size_t)s->walk_blocks * GS_NRED, "reductions");
+    dalloc(&s->red_out, GS_NRED, "reduction");
+    dalloc(&s->slots, nslots, "slots");
+    dalloc(&s->sc, nslots, "scalars");
+    dalloc(&s->prm, nslots, "params");
+    dalloc(&s->batch, 1, "batch");
+    dalloc(&s->chunk_tab, s->chunks, "chunk table");
+    dalloc(&s->tile_tab, s->sort_tiles, "tile table");
+    dalloc(&s->wblock_slot, s->walk_blocks, "walk block table");
+    // zero state with fill kernels (graph-safe, no memset nodes)
+    check(gs_launch_fill_u32((uint32_t*)s->sc, 0, sizeof(DevScalars) / 4 * nslots, s->stream),
+          "fill");
+    check(gs_launch_fill_u32((uint32_t*)s->batch, 0, sizeof(DevBatch) / 4, s->stream), "fill");
+    check(gs_launch_fill_u32((uint32_t*)s->energy, 0, P, s->stream), "fill");
+    for (int k = 0; k < 2; k++) {
+      check(gs_launch_fill_u32((uint32_t*)s->build_status[k], 0, 2ull * s->build_words,
+                               s->stream),
+            "fill");
+      check(gs_launch_fill_u32((uint32_t*)s->posm[k], 0, 4 * P, s->stream), "fill");
+      check(gs_launch_fill_u32((uint32_t*)s->velp[k], 0, 4 * P, s->stream), "fill");
+    }
+    check(hipMemcpyAsync(s->chunk_tab, chunk_tab.data(), sizeof(uint2) * s->chunks,
+                         hipMemcpyHostToDevice, s->stream),
+          "tables");
+    check(hipMemcpyAsync(s->tile_tab, tile_tab.data(), sizeof(uint32_t) * s->sort_tiles,
+                         hipMemcpyHostToDevice, s->stream),
+          "tables");
+    check(hipMemcpyAsync(s->wblock_slot, wblock_slot.data(), sizeof(uint32_t) * s->walk_blocks,
+                         hipMemcpyHostToDevice, s->stream),
+          "tables");
+    gs_params p;
+    gs_params_default(&p);
+    for (uint32_t k = 0; k < nslots; k++) {
+      pack_params(&p, 0, &s->host_prm[k]);
+      push_slot(s, k);
+      push_params(s, k);
+    }
+    sync(s);
+  });
+  if (rc != GS_OK) {
+    free_all(s);
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+void download_slot(gs_sim* s, uint32_t slot, void* dst, const void* dev_base, size_t elem,
+                   uint64_t n, const char* what) {
+  const HostSlot& h = s->hs[slot];
+  check(hipMemcpyAsync(dst, (const char*)dev_base + (size_t)h.d.poff * elem, elem * n,
+                       hipMemcpyDeviceToHost, s->stream),
+        what);
+  sync(s);
+}
+
+DevScalars read_scalars(gs_sim* s, uint32_t slot) {
+  DevScalars h;
+  check(hipMemcpyAsync(&h, s->sc + slot, sizeof h, hipMemcpyDeviceToHost, s->stream), "scalars");
+  sync(s);
+  return h;
+}
+
+void upload_particles(gs_sim* s, uint32_t slot, const float* particles, uint64_t n) {
+  HostSlot& h = s->hs[slot];
+  need(particles != nullptr || n == 0, GS_E_INVALID, "null particles");
+  need(n <= h.d.cap, GS_E_INVALID, "more particles than the slot holds");
+  if (n) {
+    // 32-byte records -> (posm, velp): two strided copies, no staging buffer
+    check(hipMemcpy2DAsync(s->posm[s->cur] + h.d.poff, 16, particles, 32, 16, n,
+                           hipMemcpyHostToDevice, s->stream),
+          "upload positions");
+    check(hipMemcpy2DAsync(s->velp[s->cur] + h.d.poff, 16, particles + 4, 32, 16, n,
+                           hipMemcpyHostToDevice, s->stream),
+          "upload velocities");
+  }
+  h.d.n = (uint32_t)n;
+  if (!h.fixed_max_nodes)  // new MortonTree(N): maxNodes = 2N + 4096 (mortonTree.js:88)
+    h.d.max_nodes = std::min<uint64_t>(2 * n + 4096, h.d.ncap);
+  push_slot(s, slot);
+  push_params(s, slot);
+  sync(s);
+  h.have_tree = h.have_force = h.have_step = h.have_codes = false;
 }
 
 }  // namespace
-
-hipError_t gs_launch_node64(gs_sim* s);
 
 extern "C" {
 
@@ -244,84 +501,50 @@ void gs_params_default(gs_params* p) {
   p->halo_count = 0;
 }
 
+static bool valid_wg(uint32_t wg) { return wg >= 64 && wg <= 1024 && (wg & (wg - 1)) == 0; }
+
 int gs_create(const gs_config* cfg, gs_sim** out) {
   if (!cfg || !out) return GS_E_INVALID;
   *out = nullptr;
   if (cfg->n == 0 || cfg->n > GS_MAX_N) return GS_E_INVALID;
-  uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
-  if (wg < 64 || wg > 1024 || (wg & 63)) return GS_E_INVALID;
+  const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
+  if (!valid_wg(wg)) return GS_E_INVALID;
   if (cfg->radix_bits != 0 && cfg->radix_bits != GS_RADIX_BITS) return GS_E_INVALID;
-  gs_sim* s = new (std::nothrow) gs_sim();
-  if (!s) return GS_E_NOMEM;
-  s->n = cfg->n;
-  s->bucket = cfg->bucket ? cfg->bucket : 16;
-  s->flags = cfg->flags;
-  s->walk_wg = wg;
-  const uint64_t pool = 2ull * cfg->n + 4096;  // mortonTree.js:88
-  s->max_nodes = cfg->max_nodes ? cfg->max_nodes : pool;
-  const uint64_t cap = std::max<uint64_t>(pool, std::min<uint64_t>(s->max_nodes, 0xfffffff0ull));
-  s->tree.cap = (uint32_t)cap;
-  s->sort_tiles = (s->n + GS_SORT_TILE - 1) / GS_SORT_TILE;
-  s->build_tiles = (uint32_t)((cap + GS_BUILD_TILE - 1) / GS_BUILD_TILE);
-  s->red_blocks = (s->n + wg - 1) / wg;
-  int dev = cfg->device;
-  int rc = guarded(s, [&] {
-    int ndev = 0;
-    check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
-    need(ndev > 0, GS_E_DEVICE, "no HIP device");
-    if (dev < 0) check(hipGetDevice(&dev), "hipGetDevice");
-    need(dev < ndev, GS_E_INVALID, "device ordinal out of range");
-    s->device = dev;
-    check(hipSetDevice(dev), "hipSetDevice");
-    hipDeviceProp_t prop;
-    check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-    need(strncmp(prop.gcnArchName, "gfx950", 6) == 0, GS_E_DEVICE,
-         "libgalaxy_hip is built for gfx950 (MI355X) only");
-    check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
-    const size_t n = s->n;
-    for (int k = 0; k < 2; k++) {
-      dalloc(&s->posm[k], n, "posm");
-      dalloc(&s->velp[k], n, "velp");
-      dalloc(&s->keys[k], n, "keys");
-      dalloc(&s->vals[k], n, "vals");
-      dalloc(&s->build_status[k], s->build_tiles, "build status");
-    }
-    dalloc(&s->codes_raw, n, "codes");
-    dalloc(&s->sort_status, (size_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX, "sort status");
-    dalloc(&s->bbox_part, (size_t)GS_BBOX_BLOCKS * 8, "bbox");
-    dalloc(&s->tree.node, cap, "nodes");
-    dalloc(&s->energy, n, "energy");
-    dalloc(&s->acc, n, "acc");
-    dalloc(&s->dens, n, "density");
-    dalloc(&s->inter, n, "interactions");
-    dalloc(&s->red_part, (size_t)s->red_blocks * GS_NRED, "reductions");
-    dalloc(&s->sc, 1, "scalars");
-    dalloc(&s->prm, 1, "params");
-    // zero state with fill kernels (graph-safe, no memset nodes)
-    check(gs_launch_fill_u32((uint32_t*)s->sc, 0, sizeof(DevScalars) / 4, s->stream), "fill");
-    check(gs_launch_fill_u32(s->energy ? (uint32_t*)s->energy : nullptr, 0, n, s->stream), "fill");
-    for (int k = 0; k < 2; k++)
-      check(gs_launch_fill_u32(s->build_status[k] ? (uint32_t*)s->build_status[k] : nullptr, 0,
-                               2ull * s->build_tiles, s->stream),
-            "fill");
-    check(gs_launch_fill_u32(s->sort_status, 0, (size_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX,
-                             s->stream),
-          "fill");
-    gs_params p;
-    gs_params_default(&p);
-    pack_params(&p, s->n, &s->host_prm);
-    check(hipMemcpyAsync(s->prm, &s->host_prm, sizeof(DevParams), hipMemcpyHostToDevice,
-                         s->stream),
-          "params");
-    check(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
-  });
-  if (rc != GS_OK) {
-    std::string msg = s->err;
-    free_all(s);
-    delete s;
-    (void)msg;
-    return rc;
+  const uint32_t bucket = cfg->bucket ? cfg->bucket : 16;
+  gs_sim* s = nullptr;
+  try {
+    if (cfg->max_nodes >= (1ull << 31)) return GS_E_INVALID;
+    s = make_pool(&cfg->n, &bucket, 1, cfg->flags, cfg->device, wg, cfg->max_nodes);
+  } catch (...) {
+    s = nullptr;
   }
+  if (!s) return GS_E_DEVICE;
+  if (cfg->max_nodes) {
+    // the leaf rule's pool; the device pool holds max(2N + 4096, max_nodes)
+    if (gs_slot_set_tree(s, 0, bucket, cfg->max_nodes) != GS_OK) {
+      gs_destroy(s);
+      return GS_E_INVALID;
+    }
+  }
+  *out = s;
+  return GS_OK;
+}
+
+int gs_create_pool(const gs_pool_config* cfg, gs_sim** out) {
+  if (!cfg || !out || !cfg->capacity) return GS_E_INVALID;
+  *out = nullptr;
+  if (cfg->nslots == 0 || cfg->nslots > GS_MAX_SLOTS) return GS_E_INVALID;
+  for (uint32_t k = 0; k < cfg->nslots; k++)
+    if (cfg->capacity[k] == 0 || cfg->capacity[k] > GS_MAX_N) return GS_E_INVALID;
+  const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
+  if (!valid_wg(wg)) return GS_E_INVALID;
+  gs_sim* s = nullptr;
+  try {
+    s = make_pool(cfg->capacity, cfg->bucket, cfg->nslots, cfg->flags, cfg->device, wg, 0);
+  } catch (...) {
+    s = nullptr;
+  }
+  if (!s) return GS_E_DEVICE;
   *out = s;
   return GS_OK;
 }
@@ -336,65 +559,114 @@ void gs_destroy(gs_sim* s) {
 
 const char* gs_last_error(const gs_sim* s) { return s ? s->err.c_str() : "null handle"; }
 
-int gs_upload(gs_sim* s, const float* particles, uint64_t n) {
+int gs_pool_info(gs_sim* s, uint32_t* nslots, uint64_t* device_bytes) {
   return guarded(s, [&] {
-    need(particles != nullptr && n == s->n, GS_E_INVALID, "upload needs exactly N records");
-    // 32-byte records -> (posm, velp): two strided copies, no staging buffer
-    check(hipMemcpy2DAsync(s->posm[s->cur], 16, particles, 32, 16, n, hipMemcpyHostToDevice,
-                           s->stream),
-          "upload positions");
-    check(hipMemcpy2DAsync(s->velp[s->cur], 16, particles + 4, 32, 16, n, hipMemcpyHostToDevice,
-                           s->stream),
-          "upload velocities");
-    check(hipStreamSynchronize(s->stream), "upload");
-    s->have_tree = s->have_force = s->have_step = false;
+    if (nslots) *nslots = s->nslots;
+    if (device_bytes) {
+      const uint64_t P = s->ptot;
+      *device_bytes = 2 * P * 32 + 4 * P * 4 + P * 4 + P * (4 + 16 + 4 + 4) +
+                      (uint64_t)s->ntot * sizeof(NodeRec) + 2ull * s->build_words * 8 +
+                      (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX * 4 +
+                      (uint64_t)s->walk_blocks * GS_NRED * 8;
+    }
   });
 }
 
-int gs_set_params(gs_sim* s, const gs_params* p) {
+int gs_upload(gs_sim* s, const float* particles, uint64_t n) {
   return guarded(s, [&] {
+    need(n == s->hs[0].d.cap, GS_E_INVALID, "upload needs exactly N records");
+    upload_particles(s, 0, particles, n);
+  });
+}
+
+int gs_slot_upload(gs_sim* s, uint32_t slot, const float* particles, uint64_t n) {
+  return guarded(s, [&] {
+    slot_of(s, slot);
+    upload_particles(s, slot, particles, n);
+  });
+}
+
+int gs_slot_clear(gs_sim* s, uint32_t slot) {
+  return guarded(s, [&] {
+    slot_of(s, slot);
+    upload_particles(s, slot, nullptr, 0);
+  });
+}
+
+int gs_slot_count(gs_sim* s, uint32_t slot, uint32_t* n) {
+  return guarded(s, [&] {
+    need(n != nullptr, GS_E_INVALID, "null output");
+    *n = slot_of(s, slot).d.n;
+  });
+}
+
+int gs_set_params(gs_sim* s, const gs_params* p) { return gs_slot_set_params(s, 0, p); }
+
+int gs_slot_set_params(gs_sim* s, uint32_t slot, const gs_params* p) {
+  return guarded(s, [&] {
+    HostSlot& h = slot_of(s, slot);
     need(p != nullptr, GS_E_INVALID, "null params");
-    pack_params(p, s->n, &s->host_prm);
-    check(hipMemcpyAsync(s->prm, &s->host_prm, sizeof(DevParams), hipMemcpyHostToDevice,
-                         s->stream),
-          "params");
-    check(hipStreamSynchronize(s->stream), "params");
-    s->have_params = true;
+    pack_params(p, h.d.n, &s->host_prm[slot]);
+    push_params(s, slot);
+    sync(s);
+    h.have_params = true;
+  });
+}
+
+int gs_slot_set_tree(gs_sim* s, uint32_t slot, uint32_t bucket, uint64_t max_nodes) {
+  return guarded(s, [&] {
+    HostSlot& h = slot_of(s, slot);
+    need(bucket >= 1, GS_E_INVALID, "bucket must be >= 1");
+    const uint64_t mn = max_nodes ? max_nodes : h.d.ncap;
+    need(mn <= h.d.ncap, GS_E_INVALID, "max_nodes larger than the slot's node pool");
+    h.d.bucket = bucket;
+    h.d.max_nodes = mn;
+    h.fixed_max_nodes = max_nodes != 0;
+    push_slot(s, slot);
+    sync(s);
+    h.have_tree = false;
   });
 }
 
 int gs_update_central_mass(gs_sim* s, float mass, uint32_t idx) {
   return guarded(s, [&] {
-    need(idx < s->n, GS_E_INVALID, "index out of range");
+    const HostSlot& h = s->hs[0];
+    need(idx < h.d.n, GS_E_INVALID, "index out of range");
     // mass is the w of posm in both ping-pong buffers (NBodyGPU.js:248-250)
     for (int k = 0; k < 2; k++)
-      check(hipMemcpyAsync((char*)(s->posm[k] + idx) + 12, &mass, 4, hipMemcpyHostToDevice,
-                           s->stream),
+      check(hipMemcpyAsync((char*)(s->posm[k] + h.d.poff + idx) + 12, &mass, 4,
+                           hipMemcpyHostToDevice, s->stream),
             "central mass");
-    check(hipStreamSynchronize(s->stream), "central mass");
+    sync(s);
   });
 }
 
 int gs_apply_permutation(gs_sim* s, const uint32_t* perm, uint64_t n) {
   return guarded(s, [&] {
-    need(perm != nullptr && n == s->n, GS_E_INVALID, "permutation needs N entries");
+    const HostSlot& h = s->hs[0];
+    need(perm != nullptr && n == h.d.n, GS_E_INVALID, "permutation needs N entries");
     std::vector<uint8_t> seen(n, 0);
     for (uint64_t i = 0; i < n; i++) {
       need(perm[i] < n && !seen[perm[i]], GS_E_INVALID, "not a permutation of [0, N)");
       seen[perm[i]] = 1;
     }
-    check(hipMemcpyAsync(s->vals[0], perm, 4 * n, hipMemcpyHostToDevice, s->stream), "perm");
-    check(gs_launch_gather(s, s->vals[0]), "gather");
-    check(hipStreamSynchronize(s->stream), "gather");
+    uint32_t* dperm = nullptr;
+    dalloc(&dperm, n, "perm");
+    hipError_t e = hipMemcpyAsync(dperm, perm, 4 * n, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = gs_launch_gather(s, 0, dperm);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dperm);
+    check(e, "gather");
     s->cur ^= 1;
-    s->have_force = false;
+    for (auto& hh : s->hs) hh.have_force = false;
   });
 }
 
 int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
                    const uint32_t* level_ranges, uint32_t num_levels) {
   return guarded(s, [&] {
-    need(nodes != nullptr && num_nodes >= 1 && num_nodes <= s->tree.cap, GS_E_INVALID,
+    HostSlot& h = s->hs[0];
+    need(nodes != nullptr && num_nodes >= 1 && num_nodes <= h.d.ncap, GS_E_INVALID,
          "node count outside [1, pool]");
     need(num_levels <= GS_MAX_LEVELS && (num_levels == 0 || level_ranges), GS_E_INVALID,
          "bad level ranges");
@@ -404,7 +676,7 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
     for (uint64_t i = 0; i < num_nodes; i++) {
       const uint32_t* w = nodes + i * 16;
       if (w[5] == 1u) {
-        need(w[6] <= s->n && w[7] <= s->n - w[6], GS_E_INVALID, "leaf range outside [0, N)");
+        need(w[6] <= h.d.n && w[7] <= h.d.n - w[6], GS_E_INVALID, "leaf range outside [0, N)");
       } else {
         int32_t prev = -1;
         for (int k = 0; k < 8; k++) {
@@ -420,31 +692,28 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
         }
       }
     }
-    if (!s->node64) dalloc(&s->node64, (size_t)s->tree.cap * 16, "node staging");
+    if (!s->node64) dalloc(&s->node64, (size_t)s->max_ncap * 16, "node staging");
     check(hipMemcpyAsync(s->node64, nodes, num_nodes * 64, hipMemcpyHostToDevice, s->stream),
           "nodes");
-    DevScalars h;
-    check(hipMemcpyAsync(&h, s->sc, sizeof h, hipMemcpyDeviceToHost, s->stream), "scalars");
-    check(hipStreamSynchronize(s->stream), "sync");
-    h.num_nodes = (uint32_t)num_nodes;
-    h.num_levels = num_levels;
+    DevScalars sc = read_scalars(s, 0);
+    sc.num_nodes = (uint32_t)num_nodes;
+    sc.num_levels = num_levels;
     for (uint32_t k = 0; k <= GS_MAX_LEVELS; k++) {
-      h.level_start[k] = k < num_levels ? level_ranges[2 * k] : 0;
-      h.level_count[k] = k < num_levels ? level_ranges[2 * k + 1] : 0;
+      sc.level_start[k] = k < num_levels ? level_ranges[2 * k] : 0;
+      sc.level_count[k] = k < num_levels ? level_ranges[2 * k + 1] : 0;
     }
-    h.status = 0;
-    check(hipMemcpyAsync(s->sc, &h, sizeof h, hipMemcpyHostToDevice, s->stream), "scalars");
+    check(hipMemcpyAsync(s->sc, &sc, sizeof sc, hipMemcpyHostToDevice, s->stream), "scalars");
     int2* dlinks = nullptr;
     dalloc(&dlinks, num_nodes, "links");
     hipError_t e = hipMemcpyAsync(dlinks, links.data(), sizeof(int2) * num_nodes,
                                   hipMemcpyHostToDevice, s->stream);
-    if (e == hipSuccess) e = gs_launch_import64(s, s->node64, dlinks, (uint32_t)num_nodes);
+    if (e == hipSuccess) e = gs_launch_import64(s, 0, s->node64, dlinks, (uint32_t)num_nodes);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(dlinks);
     check(e, "import");
     check_status(s);  // non-contiguous children are rejected here
-    s->have_tree = true;
-    s->have_force = false;
+    h.have_tree = true;
+    h.have_force = false;
   });
 }
 
@@ -454,41 +723,45 @@ int gs_encode(gs_sim* s) {
     check(gs_launch_bbox(s, s->posm[s->cur]), "bbox");
     check(gs_launch_root(s), "root");
     check(gs_launch_encode(s, s->posm[s->cur], true), "encode");
-    s->have_codes = true;
+    for (auto& h : s->hs) {
+      h.have_codes = h.d.n > 0;
+      h.have_tree = false;  // the root node slot was rewritten
+    }
   });
 }
 
 int gs_sort(gs_sim* s) {
   return guarded(s, [&] {
-    do_sort(s, true);
-    s->have_codes = true;
+    launch_sort(s, true);
+    mark_sorted(s);
+    for (auto& h : s->hs) h.have_codes = h.d.n > 0;
   });
 }
 
 int gs_refresh(gs_sim* s) {
   return guarded(s, [&] {
-    need(s->have_tree, GS_E_STATE, "refresh before any sort");
-    do_refresh(s);
+    for (auto& h : s->hs) need(!h.d.n || h.have_tree, GS_E_STATE, "refresh before any sort");
+    launch_refresh(s);
   });
 }
 
 int gs_force(gs_sim* s) {
   return guarded(s, [&] {
-    need(s->have_tree, GS_E_STATE, "force before any sort");
+    for (auto& h : s->hs) need(!h.d.n || h.have_tree, GS_E_STATE, "force before any sort");
     Stage t(s, GS_T_WALK);
-    check(gs_launch_walk(s, false, false, false), "walk");
-    s->have_force = true;
+    check(gs_launch_walk(s, false, false), "walk");
+    for (auto& h : s->hs) h.have_force = h.d.n > 0;
   });
 }
 
 int gs_step(gs_sim* s, uint32_t nsteps, uint32_t sort_every) {
   return guarded(s, [&] {
-    need(s->have_tree || sort_every > 0, GS_E_STATE, "step without a tree and sort_every = 0");
     for (uint32_t k = 0; k < nsteps; k++) {
-      Stage t(s, GS_T_STEP);
-      if (sort_every > 0 && (s->step_count % sort_every == 0 || !s->have_tree)) do_sort(s, false);
-      do_refresh(s);
-      do_walk_step(s);
+      bool missing = false;  // a slot without a tree forces a rebuild
+      for (auto& h : s->hs) missing |= h.d.n && !h.have_tree;
+      need(!missing || sort_every > 0, GS_E_STATE, "step without a tree and sort_every = 0");
+      const bool sort = missing || (sort_every > 0 && s->step_count % sort_every == 0);
+      run_step(s, sort);
       s->step_count++;
     }
   });
@@ -499,62 +772,73 @@ int gs_sync(gs_sim* s) {
 }
 
 int gs_download_particles(gs_sim* s, float* dst, uint64_t n) {
+  return gs_slot_download_particles(s, 0, dst, n);
+}
+
+int gs_slot_download_particles(gs_sim* s, uint32_t slot, float* dst, uint64_t n) {
   return guarded(s, [&] {
-    need(dst != nullptr && n == s->n, GS_E_INVALID, "download needs exactly N records");
-    check(hipMemcpy2DAsync(dst, 32, s->posm[s->cur], 16, 16, n, hipMemcpyDeviceToHost, s->stream),
-          "download positions");
-    check(hipMemcpy2DAsync(dst + 4, 32, s->velp[s->cur], 16, 16, n, hipMemcpyDeviceToHost,
+    const HostSlot& h = slot_of(s, slot);
+    need(dst != nullptr && n == h.d.n, GS_E_INVALID, "download needs exactly n records");
+    if (!n) return;
+    check(hipMemcpy2DAsync(dst, 32, s->posm[s->cur] + h.d.poff, 16, 16, n, hipMemcpyDeviceToHost,
                            s->stream),
+          "download positions");
+    check(hipMemcpy2DAsync(dst + 4, 32, s->velp[s->cur] + h.d.poff, 16, 16, n,
+                           hipMemcpyDeviceToHost, s->stream),
           "download velocities");
-    check(hipStreamSynchronize(s->stream), "download");
+    sync(s);
   });
 }
 
 int gs_read_positions(gs_sim* s, float* dst4, uint64_t n) {
   return guarded(s, [&] {
-    need(dst4 != nullptr && n == s->n, GS_E_INVALID, "read_positions needs N");
-    std::vector<float> pm((size_t)n * 4), en(n);
-    check(hipMemcpyAsync(pm.data(), s->posm[s->cur], 16 * n, hipMemcpyDeviceToHost, s->stream),
+    const HostSlot& h = s->hs[0];
+    need(dst4 != nullptr && n == h.d.n, GS_E_INVALID, "read_positions needs N");
+    // compact_positions (nbody_bh.wgsl:186-195): xyz of posm, w = energy
+    check(hipMemcpyAsync(dst4, s->posm[s->cur] + h.d.poff, 16 * n, hipMemcpyDeviceToHost,
+                         s->stream),
           "positions");
-    check(hipMemcpyAsync(en.data(), s->energy, 4 * n, hipMemcpyDeviceToHost, s->stream), "energy");
-    check(hipStreamSynchronize(s->stream), "sync");
-    for (size_t i = 0; i < n; i++) {
-      dst4[4 * i] = pm[4 * i];
-      dst4[4 * i + 1] = pm[4 * i + 1];
-      dst4[4 * i + 2] = pm[4 * i + 2];
-      dst4[4 * i + 3] = en[i];
-    }
+    check(hipMemcpy2DAsync(dst4 + 3, 16, s->energy + h.d.poff, 4, 4, n, hipMemcpyDeviceToHost,
+                           s->stream),
+          "energy");
+    sync(s);
   });
 }
 
 int gs_download_codes(gs_sim* s, uint32_t* dst, uint64_t n, int sorted) {
   return guarded(s, [&] {
-    need(dst != nullptr && n == s->n, GS_E_INVALID, "codes need N");
-    need(sorted ? s->have_tree : s->have_codes, GS_E_STATE, "no codes yet");
-    check(hipMemcpyAsync(dst, sorted ? s->keys[0] : s->codes_raw, 4 * n, hipMemcpyDeviceToHost,
-                         s->stream),
-          "codes");
-    check(hipStreamSynchronize(s->stream), "sync");
+    const HostSlot& h = s->hs[0];
+    need(dst != nullptr && n == h.d.n, GS_E_INVALID, "codes need N");
+    need(sorted ? h.have_tree : h.have_codes, GS_E_STATE, "no codes yet");
+    download_slot(s, 0, dst, sorted ? s->keys[0] : s->codes_raw, 4, n, "codes");
   });
 }
 
 int gs_download_perm(gs_sim* s, uint32_t* dst, uint64_t n) {
+  return gs_slot_download_perm(s, 0, dst, n);
+}
+
+int gs_slot_download_perm(gs_sim* s, uint32_t slot, uint32_t* dst, uint64_t n) {
   return guarded(s, [&] {
-    need(dst != nullptr && n == s->n, GS_E_INVALID, "perm needs N");
-    need(s->have_tree, GS_E_STATE, "no sort yet");
-    check(hipMemcpyAsync(dst, s->vals[0], 4 * n, hipMemcpyDeviceToHost, s->stream), "perm");
-    check(hipStreamSynchronize(s->stream), "sync");
+    const HostSlot& h = slot_of(s, slot);
+    need(dst != nullptr && n == h.d.n, GS_E_INVALID, "perm needs n");
+    need(h.have_tree, GS_E_STATE, "no sort yet");
+    download_slot(s, slot, dst, s->vals[0], 4, n, "perm");
   });
 }
 
 int gs_tree_info(gs_sim* s, uint64_t* num_nodes, uint32_t* level_ranges, uint32_t* num_levels,
                  double* root_size) {
+  return gs_slot_tree_info(s, 0, num_nodes, level_ranges, num_levels, root_size);
+}
+
+int gs_slot_tree_info(gs_sim* s, uint32_t slot, uint64_t* num_nodes, uint32_t* level_ranges,
+                      uint32_t* num_levels, double* root_size) {
   return guarded(s, [&] {
-    need(s->have_tree, GS_E_STATE, "no tree yet");
+    const HostSlot& hs = slot_of(s, slot);
+    need(hs.have_tree, GS_E_STATE, "no tree yet");
     check_status(s);
-    DevScalars h;
-    check(hipMemcpyAsync(&h, s->sc, sizeof h, hipMemcpyDeviceToHost, s->stream), "scalars");
-    check(hipStreamSynchronize(s->stream), "sync");
+    const DevScalars h = read_scalars(s, slot);
     if (num_nodes) *num_nodes = h.num_nodes;
     if (num_levels) *num_levels = h.num_levels;
     if (level_ranges)
@@ -567,56 +851,62 @@ int gs_tree_info(gs_sim* s, uint64_t* num_nodes, uint32_t* level_ranges, uint32_
 }
 
 int gs_download_nodes(gs_sim* s, uint32_t* dst, uint64_t cap, uint64_t* num_nodes) {
+  return gs_slot_download_nodes(s, 0, dst, cap, num_nodes);
+}
+
+int gs_slot_download_nodes(gs_sim* s, uint32_t slot, uint32_t* dst, uint64_t cap,
+                           uint64_t* num_nodes) {
   return guarded(s, [&] {
-    need(s->have_tree, GS_E_STATE, "no tree yet");
+    const HostSlot& hs = slot_of(s, slot);
+    need(hs.have_tree, GS_E_STATE, "no tree yet");
     check_status(s);
-    uint32_t nn = 0;
-    check(hipMemcpyAsync(&nn, &s->sc->num_nodes, 4, hipMemcpyDeviceToHost, s->stream), "nodes");
-    check(hipStreamSynchronize(s->stream), "sync");
+    const uint32_t nn = read_scalars(s, slot).num_nodes;
     if (num_nodes) *num_nodes = nn;
     need(dst != nullptr && cap >= nn, GS_E_INVALID, "node buffer too small");
-    if (!s->node64) dalloc(&s->node64, (size_t)s->tree.cap * 16, "node staging");
-    check(gs_launch_node64(s), "node64");
+    if (!s->node64) dalloc(&s->node64, (size_t)s->max_ncap * 16, "node staging");
+    check(gs_launch_node64(s, slot), "node64");
     check(hipMemcpyAsync(dst, s->node64, (size_t)nn * 64, hipMemcpyDeviceToHost, s->stream),
           "nodes");
-    check(hipStreamSynchronize(s->stream), "sync");
+    sync(s);
   });
 }
 
 int gs_download_energy(gs_sim* s, float* dst, uint64_t n) {
+  return gs_slot_download_energy(s, 0, dst, n);
+}
+
+int gs_slot_download_energy(gs_sim* s, uint32_t slot, float* dst, uint64_t n) {
   return guarded(s, [&] {
-    need(dst != nullptr && n == s->n, GS_E_INVALID, "energy needs N");
-    need(s->have_step, GS_E_STATE, "no step yet");
-    check(hipMemcpyAsync(dst, s->energy, 4 * n, hipMemcpyDeviceToHost, s->stream), "energy");
-    check(hipStreamSynchronize(s->stream), "sync");
+    const HostSlot& h = slot_of(s, slot);
+    need(dst != nullptr && n == h.d.n, GS_E_INVALID, "energy needs n");
+    need(h.have_step, GS_E_STATE, "no step yet");
+    download_slot(s, slot, dst, s->energy, 4, n, "energy");
   });
 }
 
 int gs_download_accel(gs_sim* s, float* acc4, float* density, uint64_t n) {
   return guarded(s, [&] {
-    need(n == s->n, GS_E_INVALID, "accel needs N");
-    need(s->have_force, GS_E_STATE, "no force yet");
+    const HostSlot& h = s->hs[0];
+    need(n == h.d.n, GS_E_INVALID, "accel needs N");
+    need(h.have_force, GS_E_STATE, "no force yet");
     check_status(s);
-    if (acc4)
-      check(hipMemcpyAsync(acc4, s->acc, 16 * n, hipMemcpyDeviceToHost, s->stream), "acc");
-    if (density)
-      check(hipMemcpyAsync(density, s->dens, 4 * n, hipMemcpyDeviceToHost, s->stream), "dens");
-    check(hipStreamSynchronize(s->stream), "sync");
+    if (acc4) download_slot(s, 0, acc4, s->acc, 16, n, "acc");
+    if (density) download_slot(s, 0, density, s->dens, 4, n, "density");
   });
 }
 
-int gs_reduce(gs_sim* s, double out[GS_NRED]) {
+int gs_reduce(gs_sim* s, double out[GS_NRED]) { return gs_slot_reduce(s, 0, out); }
+
+int gs_slot_reduce(gs_sim* s, uint32_t slot, double out[GS_NRED]) {
   return guarded(s, [&] {
+    const HostSlot& h = slot_of(s, slot);
     need(out != nullptr, GS_E_INVALID, "null output");
-    need(s->have_step, GS_E_STATE, "no step yet");
-    double* d = nullptr;
-    dalloc(&d, GS_NRED, "reduce");
-    hipError_t e = gs_launch_reduce_final(s, d);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(out, d, sizeof(double) * GS_NRED, hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    (void)hipFree(d);
-    check(e, "reduce");
+    need(h.have_step, GS_E_STATE, "no step yet");
+    check(gs_launch_reduce_final(s, slot, s->red_out), "reduce");
+    check(hipMemcpyAsync(out, s->red_out, sizeof(double) * GS_NRED, hipMemcpyDeviceToHost,
+                         s->stream),
+          "reduce");
+    sync(s);
   });
 }
 
@@ -636,27 +926,29 @@ int gs_stage_times(gs_sim* s, double ms[GS_NT], uint64_t counts[GS_NT], int rese
 
 int gs_count_interactions(gs_sim* s, uint64_t* total, uint32_t* per_particle) {
   return guarded(s, [&] {
-    need(s->have_tree, GS_E_STATE, "no tree yet");
+    for (auto& h : s->hs) need(!h.d.n || h.have_tree, GS_E_STATE, "no tree yet");
     uint64_t zero = 0;
-    check(hipMemcpyAsync(&s->sc->interactions, &zero, 8, hipMemcpyHostToDevice, s->stream),
+    check(hipMemcpyAsync(&s->batch->interactions, &zero, 8, hipMemcpyHostToDevice, s->stream),
           "zero");
-    check(gs_launch_walk(s, false, true, false), "count walk");
+    check(gs_launch_walk(s, false, true), "count walk");
     uint64_t t = 0;
-    check(hipMemcpyAsync(&t, &s->sc->interactions, 8, hipMemcpyDeviceToHost, s->stream), "total");
-    if (per_particle)
-      check(hipMemcpyAsync(per_particle, s->inter, 4ull * s->n, hipMemcpyDeviceToHost, s->stream),
-            "counts");
-    check(hipStreamSynchronize(s->stream), "sync");
+    check(hipMemcpyAsync(&t, &s->batch->interactions, 8, hipMemcpyDeviceToHost, s->stream),
+          "total");
+    if (per_particle) download_slot(s, 0, per_particle, s->inter, 4, s->hs[0].d.n, "counts");
+    sync(s);
     if (total) *total = t;
-    s->have_force = true;
+    for (auto& h : s->hs) h.have_force = h.d.n > 0;
   });
 }
 
 int gs_compute_morton_codes(gs_sim* s, const float* pos, uint32_t n, uint32_t stride,
                             uint32_t* codes, double* root_size) {
   return guarded(s, [&] {
-    need(pos && codes && n == s->n && stride >= 3, GS_E_INVALID, "bad morton arguments");
-    // strided host positions -> posm of the spare buffer (mass unused)
+    HostSlot& h = s->hs[0];
+    need(pos && codes && n >= 1 && n <= h.d.cap && stride >= 3, GS_E_INVALID,
+         "bad morton arguments");
+    need(s->nslots == 1, GS_E_INVALID, "host-array functions need a one-slot handle");
+    // strided host positions -> posm of the current buffer (mass unused)
     std::vector<float> tmp((size_t)n * 4);
     for (size_t i = 0; i < n; i++) {
       tmp[4 * i] = pos[i * stride];
@@ -664,30 +956,42 @@ int gs_compute_morton_codes(gs_sim* s, const float* pos, uint32_t n, uint32_t st
       tmp[4 * i + 2] = pos[i * stride + 2];
       tmp[4 * i + 3] = 0.f;
     }
-    float4* work = s->posm[s->cur ^ 1];
-    check(hipMemcpyAsync(work, tmp.data(), 16ull * n, hipMemcpyHostToDevice, s->stream), "pos");
-    check(gs_launch_bbox(s, work), "bbox");
+    check(hipMemcpyAsync(s->posm[s->cur] + h.d.poff, tmp.data(), 16ull * n, hipMemcpyHostToDevice,
+                         s->stream),
+          "pos");
+    h.d.n = n;
+    push_slot(s, 0);
+    check(gs_launch_bbox(s, s->posm[s->cur]), "bbox");
     check(gs_launch_root(s), "root");
-    check(gs_launch_encode(s, work, true), "encode");
+    check(gs_launch_encode(s, s->posm[s->cur], true), "encode");
     uint64_t sz = 0;
-    check(hipMemcpyAsync(codes, s->codes_raw, 4ull * n, hipMemcpyDeviceToHost, s->stream), "codes");
+    check(hipMemcpyAsync(codes, s->codes_raw + h.d.poff, 4ull * n, hipMemcpyDeviceToHost,
+                         s->stream),
+          "codes");
     check(hipMemcpyAsync(&sz, &s->sc->size_bits, 8, hipMemcpyDeviceToHost, s->stream), "size");
-    check(hipStreamSynchronize(s->stream), "sync");
+    sync(s);
     if (root_size) memcpy(root_size, &sz, 8);
-    s->have_tree = false;  // the root node slot was rewritten
+    h.have_tree = false;
+    h.have_codes = true;
   });
 }
 
 int gs_radix_sort_indices(gs_sim* s, const uint32_t* codes, uint32_t n, uint32_t* perm) {
   return guarded(s, [&] {
-    need(codes && perm && n == s->n, GS_E_INVALID, "bad sort arguments");
-    check(hipMemcpyAsync(s->keys[0], codes, 4ull * n, hipMemcpyHostToDevice, s->stream), "codes");
+    HostSlot& h = s->hs[0];
+    need(codes && perm && n >= 1 && n <= h.d.cap, GS_E_INVALID, "bad sort arguments");
+    need(s->nslots == 1, GS_E_INVALID, "host-array functions need a one-slot handle");
+    h.d.n = n;
+    push_slot(s, 0);
+    check(hipMemcpyAsync(s->keys[0] + h.d.poff, codes, 4ull * n, hipMemcpyHostToDevice, s->stream),
+          "codes");
     check(gs_launch_reset(s), "reset");
     check(gs_launch_hist(s), "hist");
     check(gs_launch_sort(s, false), "sort");
-    check(hipMemcpyAsync(perm, s->vals[0], 4ull * n, hipMemcpyDeviceToHost, s->stream), "perm");
-    check(hipStreamSynchronize(s->stream), "sync");
-    s->have_tree = false;
+    check(hipMemcpyAsync(perm, s->vals[0] + h.d.poff, 4ull * n, hipMemcpyDeviceToHost, s->stream),
+          "perm");
+    sync(s);
+    h.have_tree = false;
   });
 }
 
@@ -695,7 +999,10 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
                   uint32_t n, double root_size, uint32_t* nodes, uint64_t cap, uint64_t* num_nodes,
                   uint32_t* level_ranges, uint32_t* num_levels) {
   return guarded(s, [&] {
-    need(sorted_codes && pos3 && masses && n == s->n, GS_E_INVALID, "bad build arguments");
+    HostSlot& h = s->hs[0];
+    need(sorted_codes && pos3 && masses && n >= 1 && n <= h.d.cap, GS_E_INVALID,
+         "bad build arguments");
+    need(s->nslots == 1, GS_E_INVALID, "host-array functions need a one-slot handle");
     std::vector<float> pm((size_t)n * 4);
     for (size_t i = 0; i < n; i++) {
       pm[4 * i] = pos3[3 * i];
@@ -703,58 +1010,56 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
       pm[4 * i + 2] = pos3[3 * i + 2];
       pm[4 * i + 3] = masses[i];
     }
-    check(hipMemcpyAsync(s->posm[s->cur], pm.data(), 16ull * n, hipMemcpyHostToDevice, s->stream),
+    h.d.n = n;
+    push_slot(s, 0);
+    check(hipMemcpyAsync(s->posm[s->cur] + h.d.poff, pm.data(), 16ull * n, hipMemcpyHostToDevice,
+                         s->stream),
           "pos");
-    check(hipMemcpyAsync(s->keys[0], sorted_codes, 4ull * n, hipMemcpyHostToDevice, s->stream),
+    check(hipMemcpyAsync(s->keys[0] + h.d.poff, sorted_codes, 4ull * n, hipMemcpyHostToDevice,
+                         s->stream),
           "codes");
     // root segment and per-level sizes from the caller's rootSize
-    DevScalars h;
-    check(hipMemcpyAsync(&h, s->sc, sizeof h, hipMemcpyDeviceToHost, s->stream), "scalars");
-    check(hipStreamSynchronize(s->stream), "sync");
+    DevScalars sc = read_scalars(s, 0);
     double sz = root_size;
     for (int d = 0; d <= GS_MAX_LEVELS; d++) {
-      h.level_size[d] = (float)sz;
+      sc.level_size[d] = (float)sz;
       sz *= 0.5;
     }
-    memcpy(&h.size_bits, &root_size, 8);
-    h.level_start[0] = 0;
-    h.level_count[0] = 1;
-    for (int d = 1; d <= GS_MAX_LEVELS; d++) h.level_start[d] = h.level_count[d] = 0;
-    h.num_nodes = 1;
-    h.num_levels = 0;
-    h.status = 0;
-    for (int k = 0; k < 32; k++) h.tickets[k] = 0;
-    check(hipMemcpyAsync(s->sc, &h, sizeof h, hipMemcpyHostToDevice, s->stream), "scalars");
+    memcpy(&sc.size_bits, &root_size, 8);
+    for (int d = 0; d <= GS_MAX_LEVELS; d++) sc.level_start[d] = sc.level_count[d] = 0;
+    sc.level_count[0] = 1;
+    sc.num_nodes = 1;
+    sc.num_levels = 0;
+    check(hipMemcpyAsync(s->sc, &sc, sizeof sc, hipMemcpyHostToDevice, s->stream), "scalars");
+    DevBatch b;
+    memset(&b, 0, sizeof b);
+    check(hipMemcpyAsync(s->batch, &b, sizeof b, hipMemcpyHostToDevice, s->stream), "batch");
     int4 root = make_int4(0, 0, (int)gs_meta_open(0, n), -1);
-    float f0 = h.level_size[0];
+    float f0 = sc.level_size[0];
     memcpy(&root.x, &f0, 4);
-    check(hipMemcpyAsync(&s->tree.node[0].info, &root, sizeof root, hipMemcpyHostToDevice,
+    check(hipMemcpyAsync(&s->nodes[h.d.noff].info, &root, sizeof root, hipMemcpyHostToDevice,
                          s->stream),
           "root");
     check(gs_launch_build(s), "build");
     check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
-    s->have_tree = true;
-    uint64_t nn = 0;
-    uint32_t nl = 0;
+    h.have_tree = true;
     check_status(s);
-    DevScalars r;
-    check(hipMemcpyAsync(&r, s->sc, sizeof r, hipMemcpyDeviceToHost, s->stream), "scalars");
-    check(hipStreamSynchronize(s->stream), "sync");
-    nn = r.num_nodes;
-    nl = r.num_levels;
-    if (num_nodes) *num_nodes = nn;
-    if (num_levels) *num_levels = nl;
+    const DevScalars r = read_scalars(s, 0);
+    if (num_nodes) *num_nodes = r.num_nodes;
+    if (num_levels) *num_levels = r.num_levels;
     if (level_ranges)
-      for (uint32_t k = 0; k < nl && k < GS_MAX_LEVELS; k++) {
+      for (uint32_t k = 0; k < r.num_levels && k < GS_MAX_LEVELS; k++) {
         level_ranges[2 * k] = r.level_start[k];
         level_ranges[2 * k + 1] = r.level_count[k];
       }
     if (nodes) {
-      need(cap >= nn, GS_E_INVALID, "node buffer too small");
-      if (!s->node64) dalloc(&s->node64, (size_t)s->tree.cap * 16, "node staging");
-      check(gs_launch_node64(s), "node64");
-      check(hipMemcpyAsync(nodes, s->node64, nn * 64, hipMemcpyDeviceToHost, s->stream), "nodes");
-      check(hipStreamSynchronize(s->stream), "sync");
+      need(cap >= r.num_nodes, GS_E_INVALID, "node buffer too small");
+      if (!s->node64) dalloc(&s->node64, (size_t)s->max_ncap * 16, "node staging");
+      check(gs_launch_node64(s, 0), "node64");
+      check(hipMemcpyAsync(nodes, s->node64, (size_t)r.num_nodes * 64, hipMemcpyDeviceToHost,
+                           s->stream),
+            "nodes");
+      sync(s);
     }
   });
 }

@@ -17,6 +17,10 @@
 // order.  A node's children are therefore contiguous: internal nodes store
 // (firstChild, nChild, octant mask) and the 64-byte layout is produced on
 // download.  The last tile of a level writes count_{d+1}.
+//
+// One launch per level serves every slot of the pool: a block's ticket is
+// mapped to (slot, tile of the slot) through a prefix over the slots' tile
+// counts at this level; each slot has its own look-back chain and indices.
 #include "gs_internal.h"
 
 namespace {
@@ -46,38 +50,53 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ cod
 }
 
 __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
-    const uint32_t* __restrict__ codes, uint32_t n, DevTree t, DevScalars* sc, int d,
-    uint32_t bucket, uint64_t max_nodes, uint64_t* status, uint64_t* status_clear,
-    uint32_t clear_n) {
+    const uint32_t* __restrict__ keys, NodeRec* __restrict__ pool,
+    const DevSlot* __restrict__ slots, DevScalars* scs, DevBatch* batch, uint32_t nslots, int d,
+    uint64_t* status, uint64_t* status_clear, uint32_t clear_n) {
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_b[GS_BUILD_TILE][9];
   __shared__ uint32_t s_wsum[GS_BUILD_TILE / 64];
   __shared__ uint64_t s_prefix;
+  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
 
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t k = gtid; k < clear_n; k += gridDim.x * blockDim.x) status_clear[k] = 0;
 
-  const uint32_t count = sc->level_count[d];
-  const uint32_t start = sc->level_start[d];
-  if (count == 0) {
-    if (gtid == 0) {
-      sc->level_start[d + 1] = start;
-      sc->level_count[d + 1] = 0;
+  // tiles per slot at this level; slots without nodes here end their tree
+  uint32_t my_tiles = 0;
+  if (threadIdx.x < nslots) {
+    const uint32_t cnt = scs[threadIdx.x].level_count[d];
+    my_tiles = (cnt + GS_BUILD_TILE - 1) / GS_BUILD_TILE;
+    if (cnt == 0 && blockIdx.x == 0) {
+      scs[threadIdx.x].level_start[d + 1] = scs[threadIdx.x].level_start[d];
+      scs[threadIdx.x].level_count[d + 1] = 0;
     }
-    return;
   }
-  const uint32_t ntiles = (count + GS_BUILD_TILE - 1) / GS_BUILD_TILE;
-  const uint32_t next_start = start + count;
-  const uint32_t size_d = __float_as_uint(sc->level_size[d]);
-  const uint32_t size_c = __float_as_uint(sc->level_size[d + 1]);
+  gs_slot_prefix(my_tiles, nslots, s_pre);
+  const uint32_t total_tiles = s_pre[nslots];
   const int shift = 27 - 3 * d;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(&sc->tickets[4 + d], 1u);
+    if (threadIdx.x == 0) s_tile = atomicAdd(&batch->tickets[4 + d], 1u);
     __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= ntiles) break;
+    const uint32_t ticket = s_tile;
+    if (ticket >= total_tiles) break;
+    const uint32_t slot = gs_find_slot(s_pre, nslots, ticket);
+    const uint32_t tile = ticket - s_pre[slot];
+    const uint32_t ntiles = s_pre[slot + 1] - s_pre[slot];
+    const DevSlot sl = slots[slot];
+    DevScalars* sc = scs + slot;
+    NodeRec* nodes = pool + sl.noff;
+    const uint32_t* codes = keys + sl.poff;
+    const uint32_t n = sl.n, bucket = sl.bucket, cap = sl.ncap;
+    const uint64_t max_nodes = sl.max_nodes;
+    uint64_t* st_base = status + sl.btoff;
+    const uint32_t count = sc->level_count[d];
+    const uint32_t start = sc->level_start[d];
+    const uint32_t next_start = start + count;
+    const uint32_t size_d = __float_as_uint(sc->level_size[d]);
+    const uint32_t size_c = __float_as_uint(sc->level_size[d + 1]);
 
     // phase 1: octant boundaries b_0..b_7 (+ hi), 8 lanes per node
     for (int r = 0; r < GS_BUILD_TILE / 32; r++) {
@@ -86,7 +105,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
       const uint32_t local = tile * GS_BUILD_TILE + j;
       if (local < count) {
         const uint32_t idx = start + local;
-        const int4 inf = t.node[idx].info;
+        const int4 inf = nodes[idx].info;
         const uint32_t lo = (uint32_t)inf.y, cnt = gs_meta_count((uint32_t)inf.z);
         const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
         uint32_t b = lo;
@@ -136,7 +155,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const uint32_t excl_local = woff + x - nchild;
 
     if (threadIdx.x == 0) {
-      uint64_t* st = status + tile;
+      uint64_t* st = st_base + tile;
       uint64_t prefix = 0;
       if (tile == 0) {
         st_agent64(st, kPre | tile_total);
@@ -145,11 +164,11 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
         int32_t j = (int32_t)tile - 1;
         uint32_t spins = 0;
         for (;;) {
-          const uint64_t v = ld_agent64(status + j);
+          const uint64_t v = ld_agent64(st_base + j);
           const uint64_t f = v & 0xffffffff00000000ull;
           if (f == 0) {
             if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
-              atomicOr(&sc->status, GS_ST_SPIN);
+              atomicOr(&batch->status, GS_ST_SPIN);
               break;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -165,9 +184,9 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
       if (tile == ntiles - 1) {
         uint64_t total = prefix + tile_total;
         uint64_t cnt_next = total;
-        if ((uint64_t)next_start + total > t.cap) {
-          atomicOr(&sc->status, GS_ST_NODE_POOL);
-          cnt_next = t.cap > next_start ? t.cap - next_start : 0;
+        if ((uint64_t)next_start + total > cap) {
+          atomicOr(&batch->status, GS_ST_NODE_POOL);
+          cnt_next = cap > next_start ? cap - next_start : 0;
         }
         sc->level_start[d + 1] = next_start;
         sc->level_count[d + 1] = (uint32_t)cnt_next;
@@ -178,10 +197,10 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     __syncthreads();
 
     if (valid) {
-      const int4 inf = t.node[idx].info;
+      const int4 inf = nodes[idx].info;
       const int32_t skip = inf.w;
       if (leaf) {
-        t.node[idx].info = make_int4((int)size_d, inf.y,
+        nodes[idx].info = make_int4((int)size_d, inf.y,
                                      (int)gs_meta_leaf((uint32_t)d, gs_meta_count((uint32_t)inf.z)),
                                      skip);
       } else {
@@ -193,16 +212,16 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
           if (mask & (1u << k)) {
             const uint32_t b0 = s_b[threadIdx.x][k];
             const uint32_t b1 = k < 7 ? s_b[threadIdx.x][k + 1] : hi;
-            if (c < t.cap) {
+            if (c < cap) {
               // pre-order with octant 7 first: after child c comes c - 1
-              t.node[c].info = make_int4((int)size_c, (int)b0,
+              nodes[c].info = make_int4((int)size_c, (int)b0,
                                          (int)gs_meta_open((uint32_t)d + 1, b1 - b0),
                                          c > fc ? (int32_t)c - 1 : skip);
             }
             c++;
           }
         }
-        t.node[idx].info = make_int4((int)size_d, (int)fc,
+        nodes[idx].info = make_int4((int)size_d, (int)fc,
                                      (int)gs_meta_internal((uint32_t)d, mask, nchild), skip);
       }
     }
@@ -213,12 +232,12 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
 }  // namespace
 
 hipError_t gs_launch_build(gs_sim* s) {
-  uint32_t grid = s->build_tiles < 1024 ? s->build_tiles : 1024;
+  uint32_t grid = s->build_words < 1024 ? s->build_words : 1024;
   if (grid < 1) grid = 1;
   for (int d = 0; d <= GS_MAX_LEVEL; d++) {
     build_level_kernel<<<grid, GS_BUILD_TILE, 0, s->stream>>>(
-        s->keys[0], s->n, s->tree, s->sc, d, s->bucket, s->max_nodes, s->build_status[d & 1],
-        s->build_status[(d + 1) & 1], s->build_tiles);
+        s->keys[0], s->nodes, s->slots, s->sc, s->batch, s->nslots, d, s->build_status[d & 1],
+        s->build_status[(d + 1) & 1], s->build_words);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -227,12 +246,13 @@ hipError_t gs_launch_build(gs_sim* s) {
 
 // ---- expansion to the reference's 64-byte node layout (mortonTree.js:12-21)
 namespace {
-__global__ __launch_bounds__(256) void node64_kernel(DevTree t, const DevScalars* __restrict__ sc,
+__global__ __launch_bounds__(256) void node64_kernel(const NodeRec* __restrict__ nodes,
+                                                     const DevScalars* __restrict__ sc,
                                                      uint32_t* __restrict__ out) {
   const uint32_t num = sc->num_nodes;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
-    const float4 c = t.node[i].cm;
-    const int4 inf = t.node[i].info;
+    const float4 c = nodes[i].cm;
+    const int4 inf = nodes[i].info;
     uint32_t w[16];
     w[0] = __float_as_uint(c.x);
     w[1] = __float_as_uint(c.y);
@@ -260,30 +280,22 @@ __global__ __launch_bounds__(256) void node64_kernel(DevTree t, const DevScalars
     o[3] = make_uint4(w[12], w[13], w[14], w[15]);
   }
 }
-}  // namespace
-
-hipError_t gs_launch_node64(gs_sim* s) {
-  uint32_t grid = (s->tree.cap + 255) / 256;
-  if (grid > 4096) grid = 4096;
-  node64_kernel<<<grid, 256, 0, s->stream>>>(s->tree, s->sc, s->node64);
-  return hipGetLastError();
-}
 
 // ---- import of a tree in the reference's 64-byte layout (uploadBHTree) ----
-namespace {
-__global__ __launch_bounds__(256) void import64_kernel(DevTree t, DevScalars* sc,
+__global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nodes,
+                                                       DevBatch* batch,
                                                        const uint32_t* __restrict__ in,
                                                        const int2* __restrict__ links,
                                                        uint32_t num) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
     const uint4* r = (const uint4*)(in + (size_t)i * 16);
     const uint4 a = r[0], b = r[1], c = r[2], d = r[3];
-    t.node[i].cm = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
-                               __uint_as_float(a.w));
+    nodes[i].cm = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
+                              __uint_as_float(a.w));
     const int32_t skip = links[i].x;
     const uint32_t depth = (uint32_t)links[i].y;
     if (b.y == 1u) {
-      t.node[i].info = make_int4((int)b.x, (int)b.z, (int)gs_meta_leaf(depth, b.w), skip);
+      nodes[i].info = make_int4((int)b.x, (int)b.z, (int)gs_meta_leaf(depth, b.w), skip);
       continue;
     }
     const uint32_t ch[8] = {c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
@@ -292,41 +304,64 @@ __global__ __launch_bounds__(256) void import64_kernel(DevTree t, DevScalars* sc
       if ((int32_t)ch[k] >= 0) {
         if (fc == 0xffffffffu) fc = ch[k];
         // BFS trees keep a node's children contiguous in octant order
-        if (ch[k] != fc + nc) atomicOr(&sc->status, GS_ST_NODE_POOL);
+        if (ch[k] != fc + nc) atomicOr(&batch->status, GS_ST_LAYOUT);
         mask |= 1u << k;
         nc++;
       }
     }
-    t.node[i].info = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)gs_meta_internal(depth, mask, nc),
-                               skip);
+    nodes[i].info = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)gs_meta_internal(depth, mask, nc),
+                              skip);
   }
 }
 
-__global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict__ perm, uint32_t n,
+// gather through a host permutation for one slot; every other slot is copied
+// unchanged so the whole pool moves to the other ping-pong buffer
+__global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict__ perm,
+                                                     uint32_t target,
+                                                     const DevSlot* __restrict__ slots,
+                                                     const uint32_t* __restrict__ wblock_slot,
+                                                     uint32_t wg, uint64_t ptot,
                                                      const float4* __restrict__ pin,
                                                      const float4* __restrict__ qin,
                                                      float4* __restrict__ pout,
                                                      float4* __restrict__ qout) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t s = perm[i];
-    pout[i] = pin[s];
-    qout[i] = qin[s];
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ptot;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t slot = wblock_slot[p / wg];
+    const DevSlot sl = slots[slot];
+    const uint32_t local = (uint32_t)(p - sl.poff);
+    uint32_t src = local;
+    if (slot == target && local < sl.n) src = perm[local];
+    pout[p] = pin[sl.poff + src];
+    qout[p] = qin[sl.poff + src];
   }
 }
 }  // namespace
 
-hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, const int2* links, uint32_t num) {
-  uint32_t grid = (num + 255) / 256;
+hipError_t gs_launch_node64(gs_sim* s, uint32_t slot) {
+  const HostSlot& h = s->hs[slot];
+  uint32_t grid = (h.d.ncap + 255) / 256;
   if (grid > 4096) grid = 4096;
-  if (grid < 1) grid = 1;
-  import64_kernel<<<grid, 256, 0, s->stream>>>(s->tree, s->sc, words, links, num);
+  node64_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + h.d.noff, s->sc + slot, s->node64);
   return hipGetLastError();
 }
 
-hipError_t gs_launch_gather(gs_sim* s, const uint32_t* perm) {
-  uint32_t grid = (s->n + 255) / 256;
-  if (grid > 8192) grid = 8192;
-  gather_kernel<<<grid, 256, 0, s->stream>>>(perm, s->n, s->posm[s->cur], s->velp[s->cur],
-                                              s->posm[s->cur ^ 1], s->velp[s->cur ^ 1]);
+hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, const int2* links,
+                              uint32_t num) {
+  uint32_t grid = (num + 255) / 256;
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  import64_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + s->hs[slot].d.noff, s->batch, words,
+                                                links, num);
+  return hipGetLastError();
+}
+
+hipError_t gs_launch_gather(gs_sim* s, uint32_t slot, const uint32_t* perm) {
+  uint64_t g = (s->ptot + 255) / 256;
+  if (g > 8192) g = 8192;
+  gather_kernel<<<(uint32_t)g, 256, 0, s->stream>>>(perm, slot, s->slots, s->wblock_slot,
+                                                     s->walk_wg, s->ptot, s->posm[s->cur],
+                                                     s->velp[s->cur], s->posm[s->cur ^ 1],
+                                                     s->velp[s->cur ^ 1]);
   return hipGetLastError();
 }

@@ -1,16 +1,21 @@
-// Bounding box, root cube and 30-bit Morton encode (mortonTree.js:43-62).
+// Bounding box, root cube and 30-bit Morton encode (mortonTree.js:43-62),
+// one launch each over every slot of the pool.
 //
-//   bbox_kernel   per-block min/max over particles whose x+y+z is finite
-//                 (mortonTree.js:46-52); also clears the sort/build tickets,
-//                 digit histograms and onesweep status words of this step.
-//   root_kernel   one block: reduces the partials, then one lane evaluates
-//                 size = max(dx,dy,dz)*1.0001 + 1e-9 and s = 1024/size in
-//                 integer soft-double (bit-exact with the JS Number math),
-//                 the per-level node sizes f32(size * 2^-d), and the root
-//                 node's segment [0, N).
+//   bbox_kernel   per-block min/max over active particles whose x+y+z is
+//                 finite (mortonTree.js:46-52); a block covers one GS_CHUNK
+//                 of one slot (static chunk table).  Also clears this step's
+//                 sort tickets and onesweep status words.
+//   root_kernel   one block per slot: reduces the slot's partials, then one
+//                 lane evaluates size = max(dx,dy,dz)*1.0001 + 1e-9 and
+//                 s = 1024/size in integer soft-double (bit-exact with the JS
+//                 Number math), the per-level node sizes f32(size * 2^-d), and
+//                 the slot's root node covering [0, n); clears the slot's
+//                 digit histograms.
 //   encode_kernel q = min(ToInt32((p - min) * s), 1023) per axis, again in
 //                 soft-double, code = part1by2(qx) | qy<<1 | qz<<2; writes the
-//                 sort's (key, index) input and the 4 global digit histograms.
+//                 sort's (key, slot-local index) input, padding keys
+//                 0xffffffff (sorted last) up to the end of the slot's last
+//                 active chunk, and the slot's 4 digit histograms.
 // All three are HBM-streaming: 16 B read per particle for bbox, 16 B read +
 // 8 B written for encode.
 #include "gs_internal.h"
@@ -42,32 +47,39 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void reset_kernel(DevScalars* sc, uint32_t* sort_status,
-                                                    uint64_t status_words) {
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+__device__ __forceinline__ void clear_step_state(DevBatch* batch, uint32_t* sort_status,
+                                                 uint64_t status_words) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t k = g; k < status_words; k += stride) sort_status[k] = 0;
-  if (g < 32) sc->tickets[g] = 0;
-  for (uint64_t k = g; k < GS_SORT_PASSES * GS_RADIX; k += stride)
-    (&sc->sort_hist[0][0])[k] = 0;
+  if (g < 32) batch->tickets[g] = 0;
 }
 
-__global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ posm, uint32_t n,
-                                                   float* __restrict__ part, DevScalars* sc,
-                                                   uint32_t* sort_status, uint64_t status_words) {
-  // this step's sort/build bookkeeping is cleared here, one launch ahead of use
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t gstride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t k = g; k < status_words; k += gstride) sort_status[k] = 0;
-  if (g < 32) sc->tickets[g] = 0;
-  for (uint64_t k = g; k < GS_SORT_PASSES * GS_RADIX; k += gstride)
-    (&sc->sort_hist[0][0])[k] = 0;
+__global__ __launch_bounds__(256) void reset_kernel(DevBatch* batch, uint32_t* sort_status,
+                                                    uint64_t status_words, DevScalars* sc,
+                                                    uint32_t nslots) {
+  clear_step_state(batch, sort_status, status_words);
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t per = (uint64_t)GS_SORT_PASSES * GS_RADIX;
+  for (uint64_t k = g; k < (uint64_t)nslots * per; k += stride)
+    (&sc[k / per].sort_hist[0][0])[k % per] = 0;
+}
 
+__global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ posm,
+                                                   const DevSlot* __restrict__ slots,
+                                                   const uint2* __restrict__ chunk_tab,
+                                                   float* __restrict__ part, DevBatch* batch,
+                                                   uint32_t* sort_status, uint64_t status_words) {
+  clear_step_state(batch, sort_status, status_words);
+  const uint2 ct = chunk_tab[blockIdx.x];
+  const DevSlot sl = slots[ct.x];
+  const uint32_t end = min(ct.y + (uint32_t)GS_CHUNK, sl.n);
   float mnx = INFINITY, mny = INFINITY, mnz = INFINITY;
   float mxx = -INFINITY, mxy = -INFINITY, mxz = -INFINITY;
   uint32_t cnt = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float4 p = posm[i];
+  for (uint32_t i = ct.y + threadIdx.x; i < end; i += blockDim.x) {
+    const float4 p = posm[sl.poff + i];
     if (finite3(p)) {
       mnx = fminf(mnx, p.x); mxx = fmaxf(mxx, p.x);
       mny = fminf(mny, p.y); mxy = fmaxf(mxy, p.y);
@@ -79,7 +91,7 @@ __global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ po
   mxx = wave_max(mxx); mxy = wave_max(mxy); mxz = wave_max(mxz);
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
   __shared__ float s[4][7];
-  int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) {
     s[w][0] = mnx; s[w][1] = mny; s[w][2] = mnz;
     s[w][3] = mxx; s[w][4] = mxy; s[w][5] = mxz;
@@ -100,12 +112,19 @@ __global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ po
   }
 }
 
-__global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ part, uint32_t nparts,
-                                                   uint32_t n, DevScalars* sc, DevTree t) {
+__global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ part,
+                                                   const DevSlot* __restrict__ slots,
+                                                   DevScalars* scs, NodeRec* nodes) {
+  const uint32_t slot = blockIdx.x;
+  const DevSlot sl = slots[slot];
+  DevScalars* sc = scs + slot;
+  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x)
+    (&sc->sort_hist[0][0])[k] = 0;
   float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   uint32_t c = 0;
-  for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
-    const float* o = part + (size_t)b * 8;
+  for (uint32_t b = threadIdx.x; b < sl.nchunks; b += blockDim.x) {
+    if ((uint64_t)b * GS_CHUNK >= sl.n) break;  // chunks past n hold no particle
+    const float* o = part + (size_t)(sl.chunk0 + b) * 8;
     for (int j = 0; j < 3; j++) r[j] = fminf(r[j], o[j]);
     for (int j = 3; j < 6; j++) r[j] = fmaxf(r[j], o[j]);
     c += __float_as_uint(o[6]);
@@ -114,7 +133,7 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   for (int j = 3; j < 6; j++) r[j] = wave_max(r[j]);
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   __shared__ float s[4][7];
-  int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) {
     for (int j = 0; j < 6; j++) s[w][j] = r[j];
     s[w][6] = __uint_as_float(c);
@@ -137,8 +156,8 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   } else {
     uint64_t d[3];
     for (int j = 0; j < 3; j++) {
-      uint64_t mn = sd::from_f32(__float_as_uint(r[j]));
-      uint64_t mx = sd::from_f32(__float_as_uint(r[j + 3]));
+      const uint64_t mn = sd::from_f32(__float_as_uint(r[j]));
+      const uint64_t mx = sd::from_f32(__float_as_uint(r[j + 3]));
       sc->min_bits[j] = mn;
       d[j] = sd::sub(mx, mn);  // >= +0
     }
@@ -157,77 +176,92 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
     sc->level_size[dpt] = __uint_as_float(sd::to_f32(sz));
     if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
   }
-  // level 0 = the root, covering [0, N)
-  sc->level_start[0] = 0;
-  sc->level_count[0] = 1;
-  for (int k = 1; k <= GS_MAX_LEVELS; k++) {
+  // level 0 = the root, covering [0, n); an empty slot has no tree
+  for (int k = 0; k <= GS_MAX_LEVELS; k++) {
     sc->level_start[k] = 0;
     sc->level_count[k] = 0;
   }
-  sc->num_nodes = 1;
+  sc->level_count[0] = sl.n ? 1u : 0u;
+  sc->num_nodes = sl.n ? 1u : 0u;
   sc->num_levels = 0;
-  t.node[0].info = make_int4((int)__float_as_uint(sc->level_size[0]), 0,
-                             (int)gs_meta_open(0, n), -1);
+  if (sl.n)
+    nodes[sl.noff].info = make_int4((int)__float_as_uint(sc->level_size[0]), 0,
+                                    (int)gs_meta_open(0, sl.n), -1);
 }
 
 __device__ __forceinline__ uint32_t quant(float p, uint64_t mn, uint64_t s) {
   // Math.min((p - min) * s | 0, 1023); non-finite p gives NaN/Inf -> 0
   if (!__builtin_isfinite(p)) return 0;
-  uint64_t q = sd::mul(sd::sub(sd::from_f32(__float_as_uint(p)), mn), s);
-  int32_t v = sd::to_int32(q);
+  const uint64_t q = sd::mul(sd::sub(sd::from_f32(__float_as_uint(p)), mn), s);
+  const int32_t v = sd::to_int32(q);
   return (uint32_t)(v < 1023 ? v : 1023);  // part1by2 keeps the low 10 bits
 }
 
+__device__ __forceinline__ void flush_hist(uint32_t (*hist)[GS_RADIX], DevScalars* sc) {
+  __syncthreads();
+  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) {
+    const uint32_t v = (&hist[0][0])[k];
+    if (v) atomicAdd(&(&sc->sort_hist[0][0])[k], v);
+  }
+}
+
 template <bool KEEP_RAW>
-__global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ posm, uint32_t n,
+__global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ posm,
+                                                     const DevSlot* __restrict__ slots,
+                                                     const uint2* __restrict__ chunk_tab,
                                                      uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals,
-                                                     uint32_t* __restrict__ raw, DevScalars* sc) {
+                                                     uint32_t* __restrict__ raw,
+                                                     DevScalars* scs) {
+  const uint2 ct = chunk_tab[blockIdx.x];
+  const DevSlot sl = slots[ct.x];
+  if (ct.y >= sl.n) return;  // the whole chunk is past the slot's particles
+  DevScalars* sc = scs + ct.x;
   __shared__ uint32_t hist[GS_SORT_PASSES][GS_RADIX];
   for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) (&hist[0][0])[k] = 0;
   __syncthreads();
   const uint64_t mx = sc->min_bits[0], my = sc->min_bits[1], mz = sc->min_bits[2];
   const uint64_t s = sc->s_bits;
   const bool empty = sc->nfinite == 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float4 p = posm[i];
-    uint32_t code = 0;
-    if (!empty)
-      code = part1by2(quant(p.x, mx, s)) | (part1by2(quant(p.y, my, s)) << 1) |
-             (part1by2(quant(p.z, mz, s)) << 2);
-    keys[i] = code;
-    vals[i] = i;
-    if (KEEP_RAW) raw[i] = code;
+  const uint32_t end = min(ct.y + (uint32_t)GS_CHUNK, sl.cap);
+  for (uint32_t i = ct.y + threadIdx.x; i < end; i += blockDim.x) {
+    uint32_t code = 0xffffffffu;  // padding: sorts after every 30-bit code
+    if (i < sl.n) {
+      const float4 p = posm[sl.poff + i];
+      code = 0;
+      if (!empty)
+        code = part1by2(quant(p.x, mx, s)) | (part1by2(quant(p.y, my, s)) << 1) |
+               (part1by2(quant(p.z, mz, s)) << 2);
+      if (KEEP_RAW) raw[sl.poff + i] = code;
+    }
+    keys[sl.poff + i] = code;
+    vals[sl.poff + i] = i;
 #pragma unroll
     for (int ps = 0; ps < GS_SORT_PASSES; ps++)
       atomicAdd(&hist[ps][(code >> (ps * GS_RADIX_BITS)) & (GS_RADIX - 1)], 1u);
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) {
-    uint32_t v = (&hist[0][0])[k];
-    if (v) atomicAdd(&(&sc->sort_hist[0][0])[k], v);
-  }
+  flush_hist(hist, sc);
 }
 
-// digit histograms of host-supplied keys (gs_radix_sort_indices path)
+// digit histograms of host-supplied keys in slot 0 (gs_radix_sort_indices)
 __global__ __launch_bounds__(256) void hist_kernel(uint32_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals, uint32_t n,
-                                                   DevScalars* sc) {
+                                                   uint32_t* __restrict__ vals,
+                                                   const DevSlot* __restrict__ slots,
+                                                   DevScalars* scs) {
+  const DevSlot sl = slots[0];
   __shared__ uint32_t hist[GS_SORT_PASSES][GS_RADIX];
   for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) (&hist[0][0])[k] = 0;
   __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    uint32_t code = keys[i];
-    vals[i] = i;
+  const uint32_t end = min(((sl.n + GS_CHUNK - 1) / GS_CHUNK) * GS_CHUNK, sl.cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
+    if (i >= sl.n) keys[sl.poff + i] = 0xffffffffu;
+    const uint32_t code = keys[sl.poff + i];
+    vals[sl.poff + i] = i;
 #pragma unroll
     for (int ps = 0; ps < GS_SORT_PASSES; ps++)
       atomicAdd(&hist[ps][(code >> (ps * GS_RADIX_BITS)) & (GS_RADIX - 1)], 1u);
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) {
-    uint32_t v = (&hist[0][0])[k];
-    if (v) atomicAdd(&(&sc->sort_hist[0][0])[k], v);
-  }
+  flush_hist(hist, scs);
 }
 
 __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
@@ -236,80 +270,53 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t v, 
     p[i] = v;
 }
 
-// AoS-8 particle records <-> SoA (posm, velp)
-__global__ __launch_bounds__(256) void pack_soa_kernel(const float4* __restrict__ aos, uint32_t n,
-                                                       float4* __restrict__ posm,
-                                                       float4* __restrict__ velp) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    posm[i] = aos[2 * (size_t)i];
-    velp[i] = aos[2 * (size_t)i + 1];
-  }
-}
-__global__ __launch_bounds__(256) void unpack_soa_kernel(float4* __restrict__ aos, uint32_t n,
-                                                         const float4* __restrict__ posm,
-                                                         const float4* __restrict__ velp) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    aos[2 * (size_t)i] = posm[i];
-    aos[2 * (size_t)i + 1] = velp[i];
-  }
-}
-
 inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
   uint64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
   return (uint32_t)(g < cap ? g : cap);
 }
 
+inline uint64_t status_words(const gs_sim* s) {
+  return (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX;
+}
+
 }  // namespace
 
 hipError_t gs_launch_reset(gs_sim* s) {
-  uint64_t words = (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX;
-  reset_kernel<<<grid_for(words, 256, 2048), 256, 0, s->stream>>>(s->sc, s->sort_status, words);
+  reset_kernel<<<grid_for(status_words(s), 256, 2048), 256, 0, s->stream>>>(
+      s->batch, s->sort_status, status_words(s), s->sc, s->nslots);
   return hipGetLastError();
 }
 
 hipError_t gs_launch_bbox(gs_sim* s, const float4* posm) {
-  uint64_t words = (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX;
-  bbox_kernel<<<GS_BBOX_BLOCKS, 256, 0, s->stream>>>(posm, s->n, s->bbox_part, s->sc,
-                                                     s->sort_status, words);
+  bbox_kernel<<<s->chunks, 256, 0, s->stream>>>(posm, s->slots, s->chunk_tab, s->bbox_part,
+                                                s->batch, s->sort_status, status_words(s));
   return hipGetLastError();
 }
 
 hipError_t gs_launch_root(gs_sim* s) {
-  root_kernel<<<1, 256, 0, s->stream>>>(s->bbox_part, GS_BBOX_BLOCKS, s->n, s->sc, s->tree);
+  root_kernel<<<s->nslots, 256, 0, s->stream>>>(s->bbox_part, s->slots, s->sc, s->nodes);
   return hipGetLastError();
 }
 
 hipError_t gs_launch_encode(gs_sim* s, const float4* posm, bool keep_raw) {
-  uint32_t g = grid_for(s->n, 256, 4096);
   if (keep_raw)
-    encode_kernel<true><<<g, 256, 0, s->stream>>>(posm, s->n, s->keys[0], s->vals[0],
-                                                    s->codes_raw, s->sc);
+    encode_kernel<true><<<s->chunks, 256, 0, s->stream>>>(posm, s->slots, s->chunk_tab, s->keys[0],
+                                                           s->vals[0], s->codes_raw, s->sc);
   else
-    encode_kernel<false><<<g, 256, 0, s->stream>>>(posm, s->n, s->keys[0], s->vals[0],
-                                                     s->codes_raw, s->sc);
+    encode_kernel<false><<<s->chunks, 256, 0, s->stream>>>(posm, s->slots, s->chunk_tab,
+                                                            s->keys[0], s->vals[0], s->codes_raw,
+                                                            s->sc);
   return hipGetLastError();
 }
 
 hipError_t gs_launch_hist(gs_sim* s) {
-  hist_kernel<<<grid_for(s->n, 256, 4096), 256, 0, s->stream>>>(s->keys[0], s->vals[0], s->n,
-                                                                  s->sc);
+  hist_kernel<<<grid_for(s->hs[0].d.cap, 256, 4096), 256, 0, s->stream>>>(s->keys[0], s->vals[0],
+                                                                            s->slots, s->sc);
   return hipGetLastError();
 }
 
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st) {
   fill_u32_kernel<<<grid_for(n, 256, 4096), 256, 0, st>>>(p, v, n);
-  return hipGetLastError();
-}
-
-hipError_t gs_launch_pack_soa(gs_sim* s, const float* aos, int which) {
-  pack_soa_kernel<<<grid_for(s->n, 256, 4096), 256, 0, s->stream>>>(
-      (const float4*)aos, s->n, s->posm[which], s->velp[which]);
-  return hipGetLastError();
-}
-
-hipError_t gs_launch_unpack_soa(gs_sim* s, float* aos, int which) {
-  unpack_soa_kernel<<<grid_for(s->n, 256, 4096), 256, 0, s->stream>>>(
-      (float4*)aos, s->n, s->posm[which], s->velp[which]);
   return hipGetLastError();
 }

@@ -1,10 +1,20 @@
-// Internal device-side layout of one galaxy (gs_sim) and the kernel
+// Internal device-side layout of a slot pool (gs_sim) and the kernel
 // launchers shared by the translation units of libgalaxy_hip.so.
 //
-// HBM layout (structure of arrays, all 16-byte aligned):
-//   posm[2][N]  float4 (x, y, z, m)      ping-pong particle positions + mass
-//   velp[2][N]  float4 (vx, vy, vz, pad) ping-pong velocities + gas flag
-//   keys[2][N], vals[2][N]   u32         radix-sort ping-pong (code, index)
+// A gs_sim holds S galaxy slots (S = 1 for a plain galaxy).  Slot s owns a
+// fixed particle range [poff_s, poff_s + cap_s) (poff a multiple of
+// GS_SLOT_ALIGN, so no walk workgroup spans two slots) and a node pool
+// [noff_s, noff_s + ncap_s); it holds n_s <= cap_s active particles
+// (n_s = 0: empty).  Every stage is ONE launch over all slots: particle
+// kernels map blocks to slots with static tables, tree kernels map work to
+// slots with an in-LDS prefix over the slots' device-side counts.  Sizes that
+// change at run time (n_s, tree shape) live in device memory, so a captured
+// step graph stays valid as jobs enter and leave slots.
+//
+// HBM layout (structure of arrays, all 16-byte aligned), P = sum of caps:
+//   posm[2][P]  float4 (x, y, z, m)      ping-pong particle positions + mass
+//   velp[2][P]  float4 (vx, vy, vz, pad) ping-pong velocities + gas flag
+//   keys[2][P], vals[2][P]   u32         radix-sort ping-pong (code, index)
 //   node[C]     32-byte record (one s_load_dwordx8 in the walk):
 //               float4 cm   (comx, comy, comz, mass)
 //               int4   info (size bits, first | firstChild, meta, skip)
@@ -14,9 +24,10 @@
 //               skip: the node that follows this node's subtree in the
 //                     walk's pre-order (previous sibling, else the parent's
 //                     skip; -1 ends the walk)
-// with C = node capacity >= 2N + 4096 (mortonTree.js:88).  Children of a BFS
-// node are contiguous, so an internal node stores (firstChild, nChild, mask)
-// instead of 8 slots; gs_download_nodes expands to the 64-byte layout.
+// Node and particle indices inside a slot are slot-local (0-based).  A slot's
+// node pool holds >= 2 cap + 4096 nodes (mortonTree.js:88).  Children of a
+// BFS node are contiguous, so an internal node stores (firstChild, nChild,
+// mask) instead of 8 slots; gs_download_nodes expands to the 64-byte layout.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,8 +43,13 @@
 // status bits raised by kernels, checked by the host at sync points
 #define GS_ST_NODE_POOL 0x1u
 #define GS_ST_STACK 0x2u
-#define GS_ST_SPIN 0x4u   // a look-back wait hit its bound (never expected)
+#define GS_ST_SPIN 0x4u    // a look-back wait hit its bound (never expected)
+#define GS_ST_LAYOUT 0x8u  // an uploaded tree's children are not contiguous
 #define GS_SPIN_LIMIT (1u << 24)
+
+#define GS_SLOT_ALIGN 1024  // slot particle ranges start on multiples of this
+#define GS_MAX_SLOTS 256
+#define GS_CHUNK 4096       // particles per block in bbox / encode
 
 // Params uniform (nbody_bh.wgsl:19-42) in f32 plus derived per-step values.
 struct DevParams {
@@ -57,22 +73,44 @@ struct DevParams {
 #define GS_SORT_TILE (GS_SORT_BLOCK * GS_SORT_ITEMS)
 
 #define GS_BUILD_TILE 256
-#define GS_BBOX_BLOCKS 1024
 
-// Device-resident scalars of one galaxy.
+// Static layout and active size of one slot.
+struct DevSlot {
+  uint32_t n;          // active particles (0 = empty slot)
+  uint32_t cap;        // particle capacity
+  uint32_t poff;       // first particle (multiple of GS_SLOT_ALIGN)
+  uint32_t noff;       // first node of the slot's pool
+  uint32_t ncap;       // node pool size
+  uint32_t bucket;     // leaf capacity (mortonTree.js:25)
+  uint64_t max_nodes;  // leaf-rule pool size (mortonTree.js:88,127)
+  uint32_t chunk0;     // first bbox/encode block of the slot
+  uint32_t tile0;      // first sort tile of the slot
+  uint32_t btoff;      // first build look-back word of the slot
+  uint32_t wblock0;    // first walk block of the slot
+  uint32_t nwblocks;   // walk blocks of the slot
+  uint32_t nchunks;    // bbox/encode blocks = sort tiles of the slot (GS_CHUNK each)
+  uint32_t pad_[2];
+};
+
+// Device-resident scalars of one slot.
 struct DevScalars {
-  uint64_t min_bits[3];   // bbox minimum (f32 widened to binary64 bits)
-  uint64_t s_bits;        // 1024 / size  (binary64 bits)
-  uint64_t size_bits;     // rootSize     (binary64 bits)
-  uint32_t nfinite;       // particles that entered the bbox
-  uint32_t status;        // GS_ST_* bits
+  uint64_t min_bits[3];  // bbox minimum (f32 widened to binary64 bits)
+  uint64_t s_bits;       // 1024 / size  (binary64 bits)
+  uint64_t size_bits;    // rootSize     (binary64 bits)
+  uint32_t nfinite;      // particles that entered the bbox
   uint32_t num_nodes;
   uint32_t num_levels;
+  uint32_t pad_;
   uint32_t level_start[GS_MAX_LEVELS + 1];
   uint32_t level_count[GS_MAX_LEVELS + 1];
   float level_size[GS_MAX_LEVELS + 1];
-  uint32_t tickets[32];   // [0..3] sort passes, [4..20] build levels
-  uint32_t sort_hist[GS_SORT_PASSES][GS_RADIX];
+  uint32_t sort_hist[GS_SORT_PASSES][GS_RADIX];  // this slot's digit histograms
+};
+
+// Pool-wide bookkeeping.
+struct DevBatch {
+  uint32_t status;       // GS_ST_* bits
+  uint32_t tickets[32];  // [0..3] sort passes, [4..14] build levels
   uint64_t interactions;
 };
 
@@ -101,22 +139,55 @@ __host__ __device__ __forceinline__ uint32_t gs_meta_count(uint32_t m) { return 
 __host__ __device__ __forceinline__ uint32_t gs_meta_nc(uint32_t m) { return m & 0xfu; }
 __host__ __device__ __forceinline__ uint32_t gs_meta_mask(uint32_t m) { return (m >> 8) & 0xffu; }
 
-struct DevTree {
-  NodeRec* node;
-  uint32_t cap;
-};
+// Block-wide exclusive prefix over per-slot values (one thread per slot,
+// nslots <= blockDim.x <= 1024); result in s_pre[0..nslots], s_pre[nslots] =
+// total.  All threads must call it.
+__device__ __forceinline__ void gs_slot_prefix(uint32_t v, uint32_t nslots, uint32_t* s_pre) {
+  if (threadIdx.x < nslots) s_pre[threadIdx.x + 1] = v;
+  if (threadIdx.x == 0) s_pre[0] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0)  // serial: nslots <= 256
+    for (uint32_t k = 1; k <= nslots; k++) s_pre[k] += s_pre[k - 1];
+  __syncthreads();
+}
+
+// slot k with s_pre[k] <= x < s_pre[k + 1] (s_pre ascending, x < s_pre[n])
+__device__ __forceinline__ uint32_t gs_find_slot(const uint32_t* s_pre, uint32_t nslots,
+                                                 uint32_t x) {
+  uint32_t lo = 0, hi = nslots;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s_pre[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 struct StageTimer {
   int stage;
   hipEvent_t a, b;
 };
 
+// Host mirror of one slot.
+struct HostSlot {
+  DevSlot d;
+  bool have_params = false, have_tree = false, have_codes = false, have_force = false,
+       have_step = false;
+  bool fixed_max_nodes = false;  // else the leaf rule's pool follows n: 2n + 4096
+};
+
 struct gs_sim {
   int device = 0;
   hipStream_t stream = nullptr;
-  uint32_t n = 0, bucket = 16, flags = 0, walk_wg = 256;
-  uint64_t max_nodes = 0;
-  uint32_t sort_tiles = 0, build_tiles = 0;
+  uint32_t flags = 0, walk_wg = 256;
+  uint32_t nslots = 1;
+  uint64_t ptot = 0;         // sum of slot capacities (aligned)
+  uint64_t ntot = 0;         // sum of node pools
+  uint32_t chunks = 0;       // bbox/encode blocks
+  uint32_t sort_tiles = 0;   // sum over slots of capacity tiles
+  uint32_t build_words = 0;  // look-back words per build buffer
+  uint32_t walk_blocks = 0;
+  uint32_t max_ncap = 0;
 
   float4* posm[2] = {nullptr, nullptr};
   float4* velp[2] = {nullptr, nullptr};
@@ -124,50 +195,52 @@ struct gs_sim {
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* codes_raw = nullptr;
-  uint32_t* sort_status = nullptr;  // GS_SORT_PASSES x tiles x radix
+  uint32_t* sort_status = nullptr;  // GS_SORT_PASSES x sort_tiles x radix
   uint64_t* build_status[2] = {nullptr, nullptr};
-  float* bbox_part = nullptr;       // GS_BBOX_BLOCKS x 8
-  DevTree tree{};
+  float* bbox_part = nullptr;       // chunks x 8
+  NodeRec* nodes = nullptr;         // ntot records
   float* energy = nullptr;
   float4* acc = nullptr;
   float* dens = nullptr;
   uint32_t* inter = nullptr;
   double* red_part = nullptr;       // walk blocks x GS_NRED
-  uint32_t red_blocks = 0;
-  uint32_t* node64 = nullptr;       // download staging
-  DevScalars* sc = nullptr;
-  DevParams* prm = nullptr;
-  DevParams host_prm{};
+  double* red_out = nullptr;        // GS_NRED
+  uint32_t* node64 = nullptr;       // download / upload staging (largest pool)
+  DevSlot* slots = nullptr;         // nslots
+  DevScalars* sc = nullptr;         // nslots
+  DevParams* prm = nullptr;         // nslots
+  DevBatch* batch = nullptr;
+  uint2* chunk_tab = nullptr;       // chunks: (slot, local start)
+  uint32_t* tile_tab = nullptr;     // sort_tiles: slot << 16 | local tile
+  uint32_t* wblock_slot = nullptr;  // walk_blocks: slot of each walk block
 
-  bool have_params = false, have_tree = false, have_codes = false,
-       have_force = false, have_step = false;
+  std::vector<HostSlot> hs;
+  std::vector<DevParams> host_prm;
   uint64_t step_count = 0;
-
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // full step by parity
-  bool graph_with_sort[2] = {false, false};
 
   std::vector<StageTimer> timers;
   double t_ms[GS_NT] = {0};
   uint64_t t_cnt[GS_NT] = {0};
-  bool capturing = false;
+
+  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][sort]
+  bool graph_dirty = true;
 
   std::string err;
 };
 
 // ---- launchers (stream-ordered; return hipError_t of the launch) ----------
-hipError_t gs_launch_reset(gs_sim* s);
 hipError_t gs_launch_bbox(gs_sim* s, const float4* posm);
 hipError_t gs_launch_root(gs_sim* s);
 hipError_t gs_launch_encode(gs_sim* s, const float4* posm, bool keep_raw);
-hipError_t gs_launch_hist(gs_sim* s);  // histograms of keys[0] (host-supplied codes)
+hipError_t gs_launch_reset(gs_sim* s);
+hipError_t gs_launch_hist(gs_sim* s);  // histograms of host-supplied keys (slot 0)
 hipError_t gs_launch_sort(gs_sim* s, bool gather);
 hipError_t gs_launch_build(gs_sim* s);
 hipError_t gs_launch_refresh(gs_sim* s, const float4* posm);
-hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count, bool reduce);
-hipError_t gs_launch_reduce_final(gs_sim* s, double* out_dev);
-hipError_t gs_launch_node64(gs_sim* s);
-hipError_t gs_launch_import64(gs_sim* s, const uint32_t* words, const int2* links, uint32_t num);
-hipError_t gs_launch_gather(gs_sim* s, const uint32_t* perm);
+hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count);
+hipError_t gs_launch_reduce_final(gs_sim* s, uint32_t slot, double* out_dev);
+hipError_t gs_launch_node64(gs_sim* s, uint32_t slot);
+hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, const int2* links,
+                              uint32_t num);
+hipError_t gs_launch_gather(gs_sim* s, uint32_t slot, const uint32_t* perm);
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
-hipError_t gs_launch_pack_soa(gs_sim* s, const float* aos, int which);
-hipError_t gs_launch_unpack_soa(gs_sim* s, float* aos, int which);

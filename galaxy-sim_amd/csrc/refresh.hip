@@ -28,19 +28,28 @@ __device__ __forceinline__ void store_com(NodeRec* nodes, uint32_t idx, double m
                               (float)__dmul_rn(cz, inv), (float)m);
 }
 
-// every leaf of the tree: FP64 sum over its members in index order
+// every leaf of every slot: FP64 sum over its members in index order
 __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __restrict__ posm,
-                                                             NodeRec* __restrict__ nodes,
-                                                             const DevScalars* __restrict__ sc) {
-  const uint32_t num = sc->num_nodes;
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < num;
-       idx += gridDim.x * blockDim.x) {
+                                                             NodeRec* __restrict__ pool,
+                                                             const DevSlot* __restrict__ slots,
+                                                             const DevScalars* __restrict__ scs,
+                                                             uint32_t nslots) {
+  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
+  gs_slot_prefix(threadIdx.x < nslots ? scs[threadIdx.x].num_nodes : 0u, nslots, s_pre);
+  const uint32_t total = s_pre[nslots];
+  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < total;
+       f += gridDim.x * blockDim.x) {
+    const uint32_t slot = gs_find_slot(s_pre, nslots, f);
+    const DevSlot sl = slots[slot];
+    NodeRec* nodes = pool + sl.noff;
+    const uint32_t idx = f - s_pre[slot];
     const int4 inf = nodes[idx].info;
     if (!gs_meta_is_leaf((uint32_t)inf.z)) continue;
     double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     const uint32_t first = (uint32_t)inf.y, last = first + gs_meta_count((uint32_t)inf.z);
+    const float4* pp = posm + sl.poff;
     for (uint32_t j = first; j < last; j++) {
-      const float4 p = posm[j];
+      const float4 p = pp[j];
       const double pm = (double)p.w;
       m = __dadd_rn(m, pm);
       cx = __dadd_rn(cx, __dmul_rn((double)p.x, pm));
@@ -51,14 +60,20 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
   }
 }
 
-// internal nodes of one level: FP64 sum over children in octant order
-__global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict__ nodes,
-                                                            const DevScalars* __restrict__ sc,
-                                                            int d) {
-  const uint32_t start = sc->level_start[d], count = sc->level_count[d];
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count;
-       t += gridDim.x * blockDim.x) {
-    const uint32_t idx = start + t;
+// internal nodes of one level of every slot: FP64 sum over children in
+// octant order
+__global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict__ pool,
+                                                            const DevSlot* __restrict__ slots,
+                                                            const DevScalars* __restrict__ scs,
+                                                            uint32_t nslots, int d) {
+  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
+  gs_slot_prefix(threadIdx.x < nslots ? scs[threadIdx.x].level_count[d] : 0u, nslots, s_pre);
+  const uint32_t total = s_pre[nslots];
+  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < total;
+       f += gridDim.x * blockDim.x) {
+    const uint32_t slot = gs_find_slot(s_pre, nslots, f);
+    NodeRec* nodes = pool + slots[slot].noff;
+    const uint32_t idx = scs[slot].level_start[d] + (f - s_pre[slot]);
     const int4 inf = nodes[idx].info;
     if (gs_meta_is_leaf((uint32_t)inf.z)) continue;
     double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
@@ -78,19 +93,21 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
 }  // namespace
 
 hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
-  uint32_t grid = (s->tree.cap + 255) / 256;
-  if (grid > 8192) grid = 8192;
-  refresh_leaves_kernel<<<grid, 256, 0, s->stream>>>(posm, s->tree.node, s->sc);
+  uint64_t g = (s->ntot / 4 + 255) / 256;  // nodes are ~0.1-0.3 of the pool
+  if (g > 8192) g = 8192;
+  if (g < 64) g = 64;
+  refresh_leaves_kernel<<<(uint32_t)g, 256, 0, s->stream>>>(posm, s->nodes, s->slots, s->sc,
+                                                              s->nslots);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // internal nodes exist only at depths 0..9; a level's node count is bounded
-  // by 8^d and by the pool
+  // by nslots * 8^d and by the pools
   for (int d = GS_MAX_LEVEL - 1; d >= 0; d--) {
-    uint64_t bound = 1ull << (3 * d);
-    if (bound > s->tree.cap) bound = s->tree.cap;
-    uint32_t g = (uint32_t)((bound + 255) / 256);
-    if (g > 2048) g = 2048;
-    refresh_level_kernel<<<g, 256, 0, s->stream>>>(s->tree.node, s->sc, d);
+    uint64_t bound = (uint64_t)s->nslots << (3 * d);
+    if (bound > s->ntot) bound = s->ntot;
+    uint32_t gl = (uint32_t)((bound + 255) / 256);
+    if (gl > 2048) gl = 2048;
+    refresh_level_kernel<<<gl, 256, 0, s->stream>>>(s->nodes, s->slots, s->sc, s->nslots, d);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

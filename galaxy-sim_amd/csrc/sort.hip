@@ -3,15 +3,18 @@
 // identity index array is unique, so 4 passes of 8-bit digits give exactly the
 // permutation of the reference's 3 passes of 10-bit digits.
 //
-// One kernel per pass ("onesweep"):
-//   1. a block takes the next 4096-key tile in launch order (atomic ticket);
+// One kernel per pass ("onesweep") over every slot of the pool; each slot is
+// sorted on its own (slot-local keys, values and output positions):
+//   1. a block takes the next 4096-key tile in launch order (atomic ticket)
+//      and maps it to (slot, tile of the slot) through a static table; tiles
+//      past the slot's active particles exit;
 //   2. each wavefront ranks its 16 x 64 keys in order: per 64-key round, eight
 //      wave ballots give every lane the mask of lanes holding the same digit
 //      (peers), rank = running per-wave digit count + popcount(peers below);
 //   3. per digit, the tile publishes its count and looks back over earlier
-//      tiles' published counts / inclusive prefixes (decoupled look-back) to
-//      get its global output offset, added to the digit's global base from
-//      the histograms the encode kernel accumulated;
+//      tiles of its slot (decoupled look-back; a slot's first tile starts the
+//      chain) to get its output offset, added to the digit's base from the
+//      slot's histograms the encode kernel accumulated;
 //   4. keys are staged in LDS in tile-sorted order and written out as
 //      contiguous per-digit runs.
 // The last pass also gathers the particle records through the permutation
@@ -55,8 +58,9 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w
 template <bool GATHER>
 __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int pass,
-    uint32_t* status, DevScalars* sc, const float4* __restrict__ pin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int pass, uint32_t* status,
+    DevBatch* batch, const DevSlot* __restrict__ slots, const DevScalars* __restrict__ scs,
+    const uint32_t* __restrict__ tile_tab, const float4* __restrict__ pin,
     const float4* __restrict__ qin, float4* __restrict__ pout, float4* __restrict__ qout) {
   constexpr int ITEMS = GS_SORT_ITEMS;
   __shared__ uint32_t s_tile;
@@ -69,12 +73,24 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int shift = pass * GS_RADIX_BITS;
-  if (tid == 0) s_tile = atomicAdd(&sc->tickets[pass], 1u);
+  if (tid == 0) s_tile = atomicAdd(&batch->tickets[pass], 1u);
   for (int k = tid; k < (GS_SORT_BLOCK / 64) * GS_RADIX; k += GS_SORT_BLOCK) (&s_wh[0][0])[k] = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t gtile = s_tile;
+  const uint32_t tt = tile_tab[gtile];
+  const uint32_t slot = tt >> 16, tile = tt & 0xffffu;
+  const DevSlot sl = slots[slot];
+  const uint32_t active = (sl.n + GS_SORT_TILE - 1) / GS_SORT_TILE;
+  if (tile >= active) return;  // no later tile of this slot is active either
+  // keys exist up to the end of the slot's last active tile (inside the cap)
+  const uint32_t nk = min(active * (uint32_t)GS_SORT_TILE, sl.cap);
+  kin += sl.poff;
+  vin += sl.poff;
+  kout += sl.poff;
+  vout += sl.poff;
   const uint32_t base = tile * GS_SORT_TILE;
   const uint64_t below_mask = (1ull << lane) - 1;
+  const uint32_t n = nk;
 
   uint32_t key[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -109,22 +125,22 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     s_wh[k][d] = total;
     total += c[k];
   }
-  uint32_t* st = status + (size_t)tile * GS_RADIX + d;
+  uint32_t* st = status + (size_t)gtile * GS_RADIX + d;
   st_agent(st, (tile == 0 ? kPre : kAgg) | total);
 
-  const uint32_t hist_excl = block_excl_scan256(sc->sort_hist[pass][d], s_wsum);
+  const uint32_t hist_excl = block_excl_scan256(scs[slot].sort_hist[pass][d], s_wsum);
   const uint32_t tile_excl = block_excl_scan256(total, s_wsum);
 
   uint32_t excl = 0;
   if (tile > 0) {
-    int32_t j = (int32_t)tile - 1;
+    int32_t j = (int32_t)gtile - 1;  // a slot's tiles are consecutive
     uint32_t spins = 0;
     for (;;) {
       const uint32_t v = ld_agent(status + (size_t)j * GS_RADIX + d);
       const uint32_t f = v & ~kVal;
       if (f == 0) {
         if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
-          atomicOr(&sc->status, GS_ST_SPIN);
+          atomicOr(&batch->status, GS_ST_SPIN);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -162,8 +178,8 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
       kout[out] = k;
       vout[out] = v;
       if (GATHER && v < n) {
-        pout[out] = pin[v];
-        qout[out] = qin[v];
+        pout[sl.poff + out] = pin[sl.poff + v];
+        qout[sl.poff + out] = qin[sl.poff + v];
       }
     }
   }
@@ -172,18 +188,18 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
 }  // namespace
 
 hipError_t gs_launch_sort(gs_sim* s, bool gather) {
-  const uint32_t tiles = (s->n + GS_SORT_TILE - 1) / GS_SORT_TILE;
   for (int p = 0; p < GS_SORT_PASSES; p++) {
     uint32_t* status = s->sort_status + (size_t)p * s->sort_tiles * GS_RADIX;
     const int a = p & 1, b = a ^ 1;
     if (gather && p == GS_SORT_PASSES - 1)
-      onesweep_kernel<true><<<tiles, GS_SORT_BLOCK, 0, s->stream>>>(
-          s->keys[a], s->vals[a], s->keys[b], s->vals[b], s->n, p, status, s->sc,
-          s->posm[s->cur], s->velp[s->cur], s->posm[s->cur ^ 1], s->velp[s->cur ^ 1]);
+      onesweep_kernel<true><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
+          s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
+          s->tile_tab, s->posm[s->cur], s->velp[s->cur], s->posm[s->cur ^ 1],
+          s->velp[s->cur ^ 1]);
     else
-      onesweep_kernel<false><<<tiles, GS_SORT_BLOCK, 0, s->stream>>>(
-          s->keys[a], s->vals[a], s->keys[b], s->vals[b], s->n, p, status, s->sc, nullptr,
-          nullptr, nullptr, nullptr);
+      onesweep_kernel<false><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
+          s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
+          s->tile_tab, nullptr, nullptr, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

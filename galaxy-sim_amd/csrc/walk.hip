@@ -101,19 +101,26 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 template <bool EXACT, bool STEP, bool COUNT>
 __global__ __launch_bounds__(1024) void walk_kernel(
-    const float4* __restrict__ posm, const float4* __restrict__ velp,
-    float4* __restrict__ posm_out, float4* __restrict__ velp_out, float* __restrict__ energy,
-    float4* __restrict__ acc_out, float* __restrict__ dens_out, uint32_t* __restrict__ inter_out,
-    const NodeRec* __restrict__ nodes, const DevParams* __restrict__ prm, DevScalars* sc,
-    double* __restrict__ red, uint32_t n) {
+    const float4* __restrict__ posm_all, const float4* __restrict__ velp_all,
+    float4* __restrict__ posm_out_all, float4* __restrict__ velp_out_all,
+    float* __restrict__ energy_all, float4* __restrict__ acc_all, float* __restrict__ dens_all,
+    uint32_t* __restrict__ inter_all, const NodeRec* __restrict__ pool,
+    const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot,
+    const DevParams* __restrict__ prms, DevBatch* batch, double* __restrict__ red) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
 
-  // Blocks keep the hardware's round-robin spread over the 8 XCDs: every XCD
-  // then walks a mix of dense and sparse regions (contiguous Morton slices
-  // per XCD were measured slower: the dense-core slices finish last).
+  // a workgroup never spans two slots (slot ranges are GS_SLOT_ALIGN-aligned);
+  // blocks keep the hardware's round-robin spread over the 8 XCDs, so every
+  // XCD walks a mix of dense and sparse regions
   const uint32_t blk = blockIdx.x;
-  const uint32_t i = blk * blockDim.x + threadIdx.x;
+  const uint32_t slot = wblock_slot[blk];
+  const DevSlot sl = slots[slot];
+  const DevParams* __restrict__ prm = prms + slot;
+  const uint32_t n = sl.n;
+  const float4* __restrict__ posm = posm_all + sl.poff;
+  const NodeRec* __restrict__ nodes = pool + sl.noff;
+  const uint32_t i = (blk - sl.wblock0) * blockDim.x + threadIdx.x;  // slot-local particle
   const bool valid = i < n;
   const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
@@ -202,16 +209,17 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   }
 
   if (COUNT) {
-    if (valid) inter_out[i] = cnt;
+    if (valid) inter_all[sl.poff + i] = cnt;
     uint64_t s = cnt;
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0 && s) atomicAdd((unsigned long long*)&sc->interactions, (unsigned long long)s);
+    if (lane == 0 && s)
+      atomicAdd((unsigned long long*)&batch->interactions, (unsigned long long)s);
   }
 
   if (!STEP) {
     if (valid) {
-      acc_out[i] = make_float4(ax, ay, az, pot);
-      dens_out[i] = dens;
+      acc_all[sl.poff + i] = make_float4(ax, ay, az, pot);
+      dens_all[sl.poff + i] = dens;
     }
     return;
   }
@@ -219,7 +227,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   // ---- halos, kick-drift, gas drag, energy (nbody_bh.wgsl:144-180) --------
   double r_ekin = 0, r_epot = 0, r_ehalo = 0, r_px = 0, r_py = 0, r_pz = 0, r_espec = 0, r_m = 0;
   if (valid) {
-    const float4 v = velp[i];
+    const float4 v = velp_all[sl.poff + i];
     float pot_halo = 0.f;
     if (prm->halo_mass > 0.0f) {
       const float gm = G * prm->halo_mass;
@@ -252,11 +260,11 @@ __global__ __launch_bounds__(1024) void walk_kernel(
       vy = fy + (vy - fy) * k;
       vz = fz + (vz - fz) * k;
     }
-    posm_out[i] = make_float4(p.x + vx * dt, p.y + vy * dt, p.z + vz * dt, p.w);
-    velp_out[i] = make_float4(vx, vy, vz, gas ? -(1.0f + dens) : 0.0f);
+    posm_out_all[sl.poff + i] = make_float4(p.x + vx * dt, p.y + vy * dt, p.z + vz * dt, p.w);
+    velp_out_all[sl.poff + i] = make_float4(vx, vy, vz, gas ? -(1.0f + dens) : 0.0f);
     const float v2 = (vx * vx + vy * vy) + vz * vz;
     const float e = (0.5f * v2 + 0.5f * pot) + pot_halo;
-    energy[i] = e;
+    energy_all[sl.poff + i] = e;
     const double m = (double)p.w;
     r_ekin = 0.5 * m * (double)v2;
     r_epot = 0.5 * m * (double)pot;
@@ -282,12 +290,13 @@ __global__ __launch_bounds__(1024) void walk_kernel(
 }
 
 __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ red,
-                                                           uint32_t nblocks, double* out) {
+                                                           uint32_t b0, uint32_t nblocks,
+                                                           double* out) {
   __shared__ double s[4][GS_NRED];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double a[GS_NRED] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
-    for (int k = 0; k < GS_NRED; k++) a[k] += red[(size_t)b * GS_NRED + k];
+    for (int k = 0; k < GS_NRED; k++) a[k] += red[(size_t)(b0 + b) * GS_NRED + k];
   for (int k = 0; k < GS_NRED; k++) a[k] = wave_sum(a[k]);
   if (lane == 0)
     for (int k = 0; k < GS_NRED; k++) s[w][k] = a[k];
@@ -298,25 +307,24 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restr
 
 template <bool EXACT, bool STEP, bool COUNT>
 hipError_t launch(gs_sim* s) {
-  const uint32_t wg = s->walk_wg;
-  const uint32_t grid = (s->n + wg - 1) / wg;
   const int c = s->cur;
-  walk_kernel<EXACT, STEP, COUNT><<<grid, wg, 0, s->stream>>>(
+  walk_kernel<EXACT, STEP, COUNT><<<s->walk_blocks, s->walk_wg, 0, s->stream>>>(
       s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
-      s->inter, s->tree.node, s->prm, s->sc, s->red_part, s->n);
+      s->inter, s->nodes, s->slots, s->wblock_slot, s->prm, s->batch, s->red_part);
   return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count, bool /*reduce*/) {
+hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count) {
   const bool exact = (s->flags & GS_FLAG_EXACT_FORCE) != 0;
   if (count) return exact ? launch<true, false, true>(s) : launch<false, false, true>(s);
   if (integrate) return exact ? launch<true, true, false>(s) : launch<false, true, false>(s);
   return exact ? launch<true, false, false>(s) : launch<false, false, false>(s);
 }
 
-hipError_t gs_launch_reduce_final(gs_sim* s, double* out_dev) {
-  reduce_final_kernel<<<1, 256, 0, s->stream>>>(s->red_part, s->red_blocks, out_dev);
+hipError_t gs_launch_reduce_final(gs_sim* s, uint32_t slot, double* out_dev) {
+  const HostSlot& h = s->hs[slot];
+  reduce_final_kernel<<<1, 256, 0, s->stream>>>(s->red_part, h.d.wblock0, h.d.nwblocks, out_dev);
   return hipGetLastError();
 }

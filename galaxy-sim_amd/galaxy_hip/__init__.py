@@ -2,6 +2,8 @@
 C ABI of libgalaxy_hip.so (include/galaxy_hip.h).
 
   NBodyGPU            src/physics/NBodyGPU.js:14-369 (one galaxy on one GPU)
+  GalaxyPool          many galaxies in fixed-capacity slots, every stage one
+                      launch over all slots (no reference counterpart)
   compute_morton_codes, radix_sort_indices, MortonTree, sort_and_build
                       src/physics/mortonTree.js:43-224 (executed on the GPU)
 
@@ -44,6 +46,13 @@ class Config(ctypes.Structure):
                 ("max_nodes", ctypes.c_uint64), ("flags", ctypes.c_uint32),
                 ("device", ctypes.c_int32), ("walk_wg", ctypes.c_uint32),
                 ("radix_bits", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class PoolConfig(ctypes.Structure):
+    _fields_ = [("nslots", ctypes.c_uint32), ("capacity", ctypes.c_void_p),
+                ("bucket", ctypes.c_void_p), ("flags", ctypes.c_uint32),
+                ("device", ctypes.c_int32), ("walk_wg", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
 
 
 class Params(ctypes.Structure):
@@ -98,6 +107,21 @@ def lib():
         "gs_reduce": (ctypes.c_int, [_vp, _vp]),
         "gs_stage_times": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
         "gs_count_interactions": (ctypes.c_int, [_vp, ctypes.POINTER(_u64), _vp]),
+        "gs_create_pool": (ctypes.c_int, [ctypes.POINTER(PoolConfig), ctypes.POINTER(_vp)]),
+        "gs_pool_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
+        "gs_slot_upload": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
+        "gs_slot_clear": (ctypes.c_int, [_vp, _u32]),
+        "gs_slot_count": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(_u32)]),
+        "gs_slot_set_params": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(Params)]),
+        "gs_slot_set_tree": (ctypes.c_int, [_vp, _u32, _u32, _u64]),
+        "gs_slot_download_particles": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
+        "gs_slot_download_perm": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
+        "gs_slot_tree_info": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(_u64), _vp,
+                                             ctypes.POINTER(_u32),
+                                             ctypes.POINTER(ctypes.c_double)]),
+        "gs_slot_download_nodes": (ctypes.c_int, [_vp, _u32, _vp, _u64, ctypes.POINTER(_u64)]),
+        "gs_slot_download_energy": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
+        "gs_slot_reduce": (ctypes.c_int, [_vp, _u32, _vp]),
         "gs_compute_morton_codes": (ctypes.c_int, [_vp, _vp, _u32, _u32, _vp,
                                                    ctypes.POINTER(ctypes.c_double)]),
         "gs_radix_sort_indices": (ctypes.c_int, [_vp, _vp, _u32, _vp]),
@@ -124,6 +148,18 @@ def device_count():
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_params(dt, G, epsilon, theta=0.5, haloMass=0.0, haloA=1.0, gasDrag=0.0, time=0.0,
+                agingRate=25.0, sfDensity=1e9, sfEff=0.0, sfDt=0.0, haloCenters=()):
+    """The updateParams dictionary (NBodyGPU.js:253-275) as a gs_params struct."""
+    p = Params(dt=dt, G=G, epsilon=epsilon, theta=theta, halo_mass=haloMass, halo_a=haloA,
+               gas_drag=gasDrag, time=time, aging_rate=agingRate, sf_density=sfDensity,
+               sf_eff=sfEff, sf_dt=sfDt, halo_count=min(len(haloCenters), 2))
+    for h, c in enumerate(list(haloCenters)[:2]):
+        for k in range(3):
+            p.halo_centers[h][k] = float(c[k])
+    return p
 
 
 class NBodyGPU:
@@ -183,12 +219,8 @@ class NBodyGPU:
                       time=0.0, agingRate=25.0, sfDensity=1e9, sfEff=0.0, sfDt=0.0,
                       haloCenters=()):
         """NBodyGPU.updateParams (NBodyGPU.js:253-275)."""
-        p = Params(dt=dt, G=G, epsilon=epsilon, theta=theta, halo_mass=haloMass, halo_a=haloA,
-                   gas_drag=gasDrag, time=time, aging_rate=agingRate, sf_density=sfDensity,
-                   sf_eff=sfEff, sf_dt=sfDt, halo_count=min(len(haloCenters), 2))
-        for h, c in enumerate(list(haloCenters)[:2]):
-            for k in range(3):
-                p.halo_centers[h][k] = float(c[k])
+        p = make_params(dt, G, epsilon, theta, haloMass, haloA, gasDrag, time, agingRate,
+                        sfDensity, sfEff, sfDt, haloCenters)
         self._call("gs_set_params", ctypes.byref(p))
         self.params = p
 
@@ -282,6 +314,139 @@ class NBodyGPU:
         self._call("gs_count_interactions", ctypes.byref(tot),
                    _ptr(pp) if pp is not None else None)
         return (tot.value, pp) if per_particle else tot.value
+
+
+class GalaxyPool:
+    """Many galaxies on one GPU in fixed-capacity slots.
+
+    Every stage of a step is one launch over all slots and a step replays one
+    captured graph, so a pool of small galaxies runs at the launch count of one
+    galaxy.  Each slot holds 0 .. capacity particles with its own parameters
+    and leaf capacity; its results equal those of the same galaxy run alone on
+    an NBodyGPU.
+    """
+
+    def __init__(self, capacities, buckets=None, exact_force=False, timing=False, device=-1,
+                 walk_wg=0, graph=True):
+        caps = np.ascontiguousarray(capacities, dtype=np.uint32)
+        self.nslots = int(caps.size)
+        self.capacity = caps.copy()
+        bk = None if buckets is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(buckets, np.uint32), caps.shape), dtype=np.uint32)
+        cfg = PoolConfig(nslots=self.nslots, capacity=caps.ctypes.data,
+                         bucket=bk.ctypes.data if bk is not None else None,
+                         flags=(FLAG_EXACT_FORCE if exact_force else 0) |
+                         (FLAG_TIMING if timing else 0) | (0 if graph else FLAG_NO_GRAPH),
+                         device=device, walk_wg=walk_wg)
+        h = ctypes.c_void_p()
+        rc = lib().gs_create_pool(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != GS_OK:
+            raise GalaxyError(rc, "gs_create_pool failed (no usable gfx950 device, bad config "
+                              "or out of device memory)")
+        self._h = h
+        self.counts = np.zeros(self.nslots, np.int64)
+
+    _call = NBodyGPU._call
+    destroy = NBodyGPU.destroy
+    __del__ = NBodyGPU.__del__
+    __enter__ = NBodyGPU.__enter__
+    __exit__ = NBodyGPU.__exit__
+
+    def info(self):
+        ns = _u32(0)
+        nb = _u64(0)
+        self._call("gs_pool_info", ctypes.byref(ns), ctypes.byref(nb))
+        return {"nslots": ns.value, "device_bytes": nb.value}
+
+    def upload(self, slot, particles, **params):
+        """Place a galaxy (n x 8 records) in a slot; params as updateParams."""
+        particles = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 8)
+        self._call("gs_slot_upload", slot, _ptr(particles), particles.shape[0])
+        self.counts[slot] = particles.shape[0]
+        if params:
+            self.set_params(slot, **params)
+
+    def clear(self, slot):
+        self._call("gs_slot_clear", slot)
+        self.counts[slot] = 0
+
+    def count(self, slot):
+        n = _u32(0)
+        self._call("gs_slot_count", slot, ctypes.byref(n))
+        return n.value
+
+    def set_params(self, slot, **kw):
+        p = make_params(**kw)
+        self._call("gs_slot_set_params", slot, ctypes.byref(p))
+
+    def set_tree(self, slot, bucket, max_nodes=0):
+        self._call("gs_slot_set_tree", slot, bucket, max_nodes)
+
+    def sort(self):
+        self._call("gs_sort")
+
+    def refresh(self):
+        self._call("gs_refresh")
+
+    def force(self):
+        self._call("gs_force")
+
+    def step(self, nsteps=1, sort_every=1):
+        self._call("gs_step", nsteps, sort_every)
+
+    def sync(self):
+        self._call("gs_sync")
+
+    def particles(self, slot):
+        n = int(self.counts[slot])
+        out = np.empty((n, 8), np.float32)
+        if n:
+            self._call("gs_slot_download_particles", slot, _ptr(out), n)
+        return out
+
+    def perm(self, slot):
+        n = int(self.counts[slot])
+        out = np.empty(n, np.uint32)
+        self._call("gs_slot_download_perm", slot, _ptr(out), n)
+        return out
+
+    def tree_info(self, slot):
+        nn = _u64(0)
+        lr = np.zeros(32, np.uint32)
+        nl = _u32(0)
+        rs = ctypes.c_double(0)
+        self._call("gs_slot_tree_info", slot, ctypes.byref(nn), _ptr(lr), ctypes.byref(nl),
+                   ctypes.byref(rs))
+        return {"num_nodes": nn.value, "num_levels": nl.value, "root_size": rs.value,
+                "level_ranges": [[int(lr[2 * k]), int(lr[2 * k + 1])]
+                                 for k in range(min(nl.value, 16))]}
+
+    def nodes(self, slot):
+        info = self.tree_info(slot)
+        out = np.empty(max(info["num_nodes"], 1) * 16, np.uint32)
+        nn = _u64(0)
+        self._call("gs_slot_download_nodes", slot, _ptr(out), info["num_nodes"],
+                   ctypes.byref(nn))
+        return out[: nn.value * 16]
+
+    def energy(self, slot):
+        n = int(self.counts[slot])
+        out = np.empty(n, np.float32)
+        self._call("gs_slot_download_energy", slot, _ptr(out), n)
+        return out
+
+    def reduce(self, slot):
+        out = np.zeros(8, np.float64)
+        self._call("gs_slot_reduce", slot, _ptr(out))
+        return dict(zip(RED_NAMES, out.tolist()))
+
+    def stage_times(self, reset=False):
+        return NBodyGPU.stage_times(self, reset)
+
+    def count_interactions(self):
+        tot = _u64(0)
+        self._call("gs_count_interactions", ctypes.byref(tot), None)
+        return tot.value
 
 
 # ---- mortonTree.js surface (device-executed on host arrays) ----------------

@@ -15,8 +15,9 @@
  *   - Every function returns GS_OK (0) or a negative GS_E* status; the
  *     handle's gs_last_error() string says what went wrong.  No C++ exception
  *     crosses this boundary.
- *   - A handle owns one HIP stream and all device state of one galaxy.  It is
- *     single-threaded; distinct handles may be driven from distinct threads.
+ *   - A handle owns one HIP stream and all device state of one galaxy, or of
+ *     a pool of galaxy slots (gs_create_pool).  It is single-threaded;
+ *     distinct handles may be driven from distinct threads.
  *   - Host buffers are caller-owned and copied at call time (the reference's
  *     writeBuffer / mapAsync semantics).  Calls are stream-ordered and return
  *     once work is queued, except the gs_download_* / gs_read_* / gs_sync /
@@ -59,11 +60,11 @@ typedef struct gs_config {
   uint32_t n;         /* particles, N (NBodyGPU.js:17); 1 .. 2^27 - 1 */
   uint32_t bucket;    /* leaf capacity BUCKET (mortonTree.js:25); 0 -> 16 */
   uint64_t max_nodes; /* leaf-rule pool size maxNodes (mortonTree.js:88,127);
-                         0 -> 2n + 4096.  The device pool always holds at
-                         least 2n + 4096 nodes. */
+                         0 -> 2n + 4096.  The device pool holds
+                         max(2n + 4096, max_nodes) nodes; at most 2^31 - 1. */
   uint32_t flags;     /* GS_FLAG_* */
   int32_t device;     /* HIP device ordinal, -1 = current device */
-  uint32_t walk_wg;   /* walk workgroup size, multiple of 64 in [64, 1024];
+  uint32_t walk_wg;   /* walk workgroup size, a power of two in [64, 1024];
                          0 -> tuned default */
   uint32_t radix_bits; /* sort digit width: 0 -> 8 (only 8 is built) */
   uint32_t reserved[4];
@@ -184,6 +185,54 @@ int gs_stage_times(gs_sim* sim, double ms[GS_NT], uint64_t counts[GS_NT],
  * positions, counted on the device by a counting walk; per-particle counts
  * go to per_particle when it is not NULL. */
 int gs_count_interactions(gs_sim* sim, uint64_t* total, uint32_t* per_particle);
+
+/* ---- slot pools: many galaxies stepped by the same launches -------------
+ * No counterpart in the reference, which runs one galaxy per NBodyGPU.  A
+ * pool holds nslots slots with fixed capacities; each slot holds 0 .. cap
+ * particles with its own params and tree shape, and every stage (encode,
+ * sort, build, refresh, walk) is one launch over all slots, replayed as one
+ * captured graph per step.  Slot results equal those of the same galaxy on a
+ * handle of its own.  The single-galaxy calls above act on slot 0 (gs_upload
+ * also requires n == its capacity); gs_encode, gs_sort, gs_refresh, gs_force,
+ * gs_step and gs_count_interactions act on every occupied slot. */
+typedef struct gs_pool_config {
+  uint32_t nslots;           /* 1 .. 256 */
+  const uint32_t* capacity;  /* nslots capacities, each 1 .. 2^27 - 1 */
+  const uint32_t* bucket;    /* nslots leaf capacities, NULL or 0 -> 16 */
+  uint32_t flags;            /* GS_FLAG_* */
+  int32_t device;            /* HIP device ordinal, -1 = current device */
+  uint32_t walk_wg;          /* as gs_config.walk_wg */
+  uint32_t reserved[5];
+} gs_pool_config;
+
+int gs_create_pool(const gs_pool_config* cfg, gs_sim** out);
+/* Slot count and device bytes held by the pool. */
+int gs_pool_info(gs_sim* sim, uint32_t* nslots, uint64_t* device_bytes);
+/* gs_upload into one slot: n (0 .. capacity) records; n = 0 empties it. */
+int gs_slot_upload(gs_sim* sim, uint32_t slot, const float* particles,
+                   uint64_t n);
+int gs_slot_clear(gs_sim* sim, uint32_t slot);
+int gs_slot_count(gs_sim* sim, uint32_t slot, uint32_t* n);
+/* gs_set_params for one slot. */
+int gs_slot_set_params(gs_sim* sim, uint32_t slot, const gs_params* p);
+/* Leaf capacity and leaf-rule pool size of one slot (max_nodes <= the
+ * slot's node pool, 2 capacity + 4096; 0 -> 2n + 4096 of each upload, as
+ * new MortonTree(n) sizes it). */
+int gs_slot_set_tree(gs_sim* sim, uint32_t slot, uint32_t bucket,
+                     uint64_t max_nodes);
+/* Per-slot forms of the downloads above. */
+int gs_slot_download_particles(gs_sim* sim, uint32_t slot, float* dst,
+                               uint64_t n);
+int gs_slot_download_perm(gs_sim* sim, uint32_t slot, uint32_t* dst,
+                          uint64_t n);
+int gs_slot_tree_info(gs_sim* sim, uint32_t slot, uint64_t* num_nodes,
+                      uint32_t* level_ranges, uint32_t* num_levels,
+                      double* root_size);
+int gs_slot_download_nodes(gs_sim* sim, uint32_t slot, uint32_t* dst,
+                           uint64_t cap, uint64_t* num_nodes);
+int gs_slot_download_energy(gs_sim* sim, uint32_t slot, float* dst,
+                            uint64_t n);
+int gs_slot_reduce(gs_sim* sim, uint32_t slot, double out[GS_NRED]);
 
 /* ---- mortonTree.js on host arrays (device-executed, copies in and out) --- */
 /* computeMortonCodes(pos, N, stride, codes) -> rootSize (mortonTree.js:43). */
