@@ -5,6 +5,7 @@ on one MI355X, headline workload = collision of two 2,097,152-particle disks
 step.  Prints ONE JSON line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+                  [--no-batch] [--no-serve] [--only headline|batch|serve]
 
 A step is gs_step(sim, 1, 1): bbox + Morton encode, 4 radix passes (last one
 gathers the particle records), 11 BFS build levels, FP64 refresh, walk with
@@ -12,6 +13,15 @@ halos + kick-drift + FP64 reductions.  Inputs are resident in HBM before the
 timed region; every step is bracketed by a stream synchronisation so the
 per-step latencies give p50/p99.  Stage durations come from HIP events the
 library records on its own stream around each stage.
+
+The same line carries two more objects (skipped with --no-batch/--no-serve):
+  batched  64 galaxies of N log-uniform in [4096, 262144] stepped as one slot
+           pool (one graph replay per step), walk workgroup / bucket / digit
+           width from the autotuner, against the same galaxies stepped with
+           per-galaxy launches;
+  served   a Poisson job stream (20 jobs/s, N log-uniform in [8192, 2^20],
+           200 steps each) served by continuous batching in real time, with
+           step p50/p99 and the one-job-at-a-time throughput beside it.
 """
 import argparse
 import json
@@ -59,6 +69,177 @@ def cpu_baseline(data, n, theta, eps, bucket, sample):
     }
 
 
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def _timed_steps(obj, steps):
+    """Per-step wall latencies (s) of obj.step(1, sort_every=1) + sync."""
+    lat = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        obj.step(1, sort_every=1)
+        obj.sync()
+        lat.append(time.perf_counter() - t0)
+    return np.array(lat)
+
+
+def bench_batch(args):
+    """64 galaxies of uneven N stepped as one slot pool (one captured graph
+    per step, every stage one launch over all galaxies), against the plain
+    per-galaxy form: each galaxy on its own handle with its own launches,
+    one galaxy after another, and (second plain form) all 64 handles queued
+    on their own streams before one synchronisation."""
+    import galaxy_hip
+    from galaxy_hip import autotune, ic
+    c = ic.CONFIGS["batch64"]
+    gals = ic.batch64(c["count"], c["lo"], c["hi"])
+    datas = [g for _, g in gals]
+    sizes = [d.shape[0] for d in datas]
+    total_n = int(sum(sizes))
+    params = dict(dt=c["dt"], G=1.0, epsilon=c["epsilon"], theta=c["theta"])
+    log(f"batch64: {len(datas)} galaxies, N total {total_n}, min {min(sizes)}, max {max(sizes)}")
+    t0 = time.perf_counter()
+    tuned = autotune.tune(datas, params, log=log)
+    t_tune = time.perf_counter() - t0
+    wg, bucket, radix = tuned["walk_wg"], tuned["bucket"], tuned["radix_bits"]
+    log(f"autotune: wg={wg} bucket={bucket} radix={radix} ({len(tuned['trials'])} trials, "
+        f"{t_tune:.1f} s)")
+
+    pool = galaxy_hip.GalaxyPool(sizes, buckets=[bucket] * len(sizes), walk_wg=wg,
+                                 radix_bits=radix)
+    try:
+        for k, d in enumerate(datas):
+            pool.upload(k, d, **params)
+        pool.step(args.warmup, sort_every=1)
+        pool.sync()
+        t0 = time.perf_counter()
+        lat = _timed_steps(pool, args.batch_steps)
+        t_pool = time.perf_counter() - t0
+        inter = pool.count_interactions()
+    finally:
+        pool.destroy()
+    pooled = total_n * args.batch_steps / t_pool
+    log(f"pooled: {t_pool / args.batch_steps * 1e3:.3f} ms/step, {pooled / 1e6:.1f} M particle-steps/s")
+
+    # plain form 1: one galaxy after another, each on its own handle
+    ksteps = args.plain_steps
+    t_seq = 0.0
+    for d in datas:
+        sim = galaxy_hip.NBodyGPU(d, bucket=bucket, walk_wg=wg, radix_bits=radix)
+        try:
+            sim.update_params(**params)
+            sim.step(1, sort_every=1)
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(ksteps, sort_every=1)
+            sim.sync()
+            t_seq += (time.perf_counter() - t0) / ksteps
+        finally:
+            sim.destroy()
+    plain_seq = total_n / t_seq
+    log(f"plain sequential: {t_seq * 1e3:.3f} ms per round of 64, {plain_seq / 1e6:.1f} M/s")
+
+    # plain form 2: 64 handles, each step queued on its own stream
+    sims = []
+    try:
+        for d in datas:
+            sim = galaxy_hip.NBodyGPU(d, bucket=bucket, walk_wg=wg, radix_bits=radix)
+            sim.update_params(**params)
+            sims.append(sim)
+        for sim in sims:
+            sim.step(1, sort_every=1)
+        for sim in sims:
+            sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(ksteps):
+            for sim in sims:
+                sim.step(1, sort_every=1)
+            for sim in sims:
+                sim.sync()
+        t_conc = (time.perf_counter() - t0) / ksteps
+    finally:
+        for sim in sims:
+            sim.destroy()
+    plain_conc = total_n / t_conc
+    log(f"plain 64 streams: {t_conc * 1e3:.3f} ms per round, {plain_conc / 1e6:.1f} M/s")
+    lat_ms = lat * 1e3
+    best_plain = max(plain_seq, plain_conc)
+    return {
+        "workload": "batch64: 64 galaxies, N log-uniform in [4096, 262144] (seeded), "
+                    "disk / Plummer a=1 50/50, theta=0.6, eps=0.01, dt=0.005, rebuild every step",
+        "n_total": total_n, "galaxies": len(sizes),
+        "value": pooled, "unit": "particle-steps/s",
+        "ms_per_step": t_pool / args.batch_steps * 1e3,
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "steps": args.batch_steps,
+        "interactions_per_s": inter / (t_pool / args.batch_steps),
+        "plain_sequential": plain_seq, "plain_streams": plain_conc,
+        "speedup_vs_plain": pooled / best_plain,
+        "speedup_vs_plain_sequential": pooled / plain_seq,
+        "autotune": {"walk_wg": wg, "bucket": bucket, "radix_bits": radix,
+                     "trials": len(tuned["trials"]), "seconds": t_tune,
+                     "space": {k: list(v) for k, v in autotune.DEFAULT_SPACE.items()}},
+    }
+
+
+def bench_serve(args, tuned):
+    """Poisson job stream served by continuous batching (GalaxyServer) in
+    real time, against the plain form: each job alone, one after another."""
+    import concurrent.futures as cf
+
+    import galaxy_hip
+    from galaxy_hip import ic, serve
+    c = ic.CONFIGS["serve"]
+    jobs = ic.serve_jobs(args.serve_jobs, rate=c["rate"], lo=c["lo"], hi=c["hi"],
+                         steps=args.serve_steps)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(8) as ex:
+        records = list(ex.map(ic.job_records, jobs))
+    log(f"serve: {len(jobs)} jobs generated in {time.perf_counter() - t0:.1f} s, "
+        f"N total {sum(j['n'] for j in jobs)}, last arrival {jobs[-1]['arrival']:.2f} s")
+    params = dict(dt=c["dt"], G=1.0, epsilon=c["epsilon"], theta=c["theta"])
+    caps = serve.slot_classes(c["lo"], c["hi"], args.serve_per_class)
+    wg = tuned.get("walk_wg", 0)
+    radix = tuned.get("radix_bits", 0)
+    with serve.GalaxyServer(caps, hbm_budget=c["hbm_budget"],
+                            resident_limit=args.serve_resident, bucket=16, walk_wg=wg,
+                            radix_bits=radix, keep_results=False) as srv:
+        js = [serve.Job(j["id"], r, params, j["steps"], j["arrival"]) for j, r in zip(jobs, records)]
+        summ = srv.run(js, realtime=True, log=log)
+    log(f"served: {summ['throughput'] / 1e6:.1f} M particle-steps/s, step p50 "
+        f"{summ['step_p50_ms']:.2f} ms p99 {summ['step_p99_ms']:.2f} ms")
+    # plain: every job alone on its own handle, timed over a few steps
+    t_plain = 0.0
+    work = 0
+    for j, r in zip(jobs, records):
+        sim = galaxy_hip.NBodyGPU(r, bucket=16, walk_wg=wg, radix_bits=radix)
+        try:
+            sim.update_params(**params)
+            sim.step(1, sort_every=1)
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(args.plain_steps, sort_every=1)
+            sim.sync()
+            t_plain += (time.perf_counter() - t0) / args.plain_steps * j["steps"]
+            work += j["n"] * j["steps"]
+        finally:
+            sim.destroy()
+    plain = work / t_plain
+    log(f"served plain (one job at a time): {plain / 1e6:.1f} M particle-steps/s")
+    summ.update({
+        "workload": "serve: Poisson arrivals 20 jobs/s, N log-uniform in [8192, 1048576], "
+                    f"{args.serve_steps} steps per job, disk / Plummer / collision uniform, "
+                    "theta=0.5, eps=0.01, dt=0.005, 200 GB HBM budget",
+        "value": summ["throughput"], "unit": "particle-steps/s",
+        "resident_limit": args.serve_resident,
+        "plain_one_at_a_time": plain, "speedup_vs_plain": summ["throughput"] / plain,
+        "p99_over_p50": summ["step_p99_ms"] / max(summ["step_p50_ms"], 1e-9),
+    })
+    return summ
+
+
 def copy_probe(mib):
     """Device-to-device copy of a known size (hipMemcpy), the calibration
     kernel for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -89,6 +270,17 @@ def main():
     ap.add_argument("--lib", default=None, help="path of the libgalaxy_hip.so to load")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-galaxy batched run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served job stream")
+    ap.add_argument("--only", choices=("headline", "batch", "serve"), default=None)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-steps", type=int, default=200)
+    ap.add_argument("--serve-resident", type=int, default=8 << 20,
+                    help="resident particle limit of the server")
+    ap.add_argument("--serve-per-class", type=int, default=8,
+                    help="slots per power-of-two capacity class")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -100,10 +292,34 @@ def main():
 
     if args.lib:
         os.environ["GALAXY_HIP_LIB"] = args.lib
+    if args.only == "batch":
+        print(json.dumps(bench_batch(args)))
+        return
+    if args.only == "serve":
+        print(json.dumps(bench_serve(args, {})))
+        return
+    out = bench_headline(args)
+    tuned = {}
+    if not (args.no_batch or args.only == "headline"):
+        try:
+            out["batched"] = bench_batch(args)
+            tuned = out["batched"]["autotune"]
+        except Exception as e:  # the headline line is still printed
+            out["batched"] = {"error": f"{type(e).__name__}: {e}"}
+    if not (args.no_serve or args.only == "headline"):
+        try:
+            out["served"] = bench_serve(args, tuned)
+        except Exception as e:
+            out["served"] = {"error": f"{type(e).__name__}: {e}"}
+    print(json.dumps(out))
+
+
+def bench_headline(args):
     import galaxy_hip
     from galaxy_hip import ic
 
     n, theta, eps, bucket = args.n, 0.5, 0.005, 32
+    log(f"headline: collision_4m N={n}")
     data = ic.make("collision_4m", n=n)
     sim = galaxy_hip.NBodyGPU(data, bucket=bucket, timing=True, exact_force=args.exact)
     sim.update_params(dt=0.005, G=1.0, epsilon=eps, theta=theta)
@@ -168,11 +384,13 @@ def main():
         "interactions_per_s": inter / (ms_per_step * 1e-3),
         "cpu_baseline": None,
     }
+    log(f"headline: {out['value'] / 1e6:.1f} M particle-steps/s, walk {walk_avg_ms:.3f} ms")
     if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on the host cores)")
         cb = cpu_baseline(data, n, theta, eps, bucket, min(args.cpu_sample, n))
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":

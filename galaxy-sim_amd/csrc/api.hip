@@ -294,10 +294,11 @@ uint32_t align_up(uint64_t v, uint32_t a) { return (uint32_t)((v + a - 1) / a * 
 // Allocates a pool of nslots slots (caps[k] particles, buckets[k] leaf size,
 // a node pool of max(2 caps[k] + 4096, max_nodes) nodes).
 gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots, uint32_t flags,
-                  int device, uint32_t wg, uint64_t max_nodes) {
+                  int device, uint32_t wg, uint32_t radix_bits, uint64_t max_nodes) {
   gs_sim* s = new gs_sim();
   s->flags = flags;
   s->walk_wg = wg;
+  s->radix_bits = radix_bits;
   s->nslots = nslots;
   s->hs.resize(nslots);
   s->host_prm.resize(nslots);
@@ -367,7 +368,7 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
       dalloc(&s->build_status[k], s->build_words, "build status");
     }
     dalloc(&s->codes_raw, P, "codes");
-    dalloc(&s->sort_status, (size_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX, "sort status");
+    dalloc(&s->sort_status, (size_t)GS_HIST_WORDS * s->sort_tiles, "sort status");
     dalloc(&s->bbox_part, (size_t)s->chunks * 8, "bbox");
     dalloc(&s->nodes, s->ntot, "nodes");
     dalloc(&s->energy, P, "energy");
@@ -441,15 +442,7 @@ void upload_particles(gs_sim* s, uint32_t slot, const float* particles, uint64_t
   HostSlot& h = s->hs[slot];
   need(particles != nullptr || n == 0, GS_E_INVALID, "null particles");
   need(n <= h.d.cap, GS_E_INVALID, "more particles than the slot holds");
-  if (n) {
-    // 32-byte records -> (posm, velp): two strided copies, no staging buffer
-    check(hipMemcpy2DAsync(s->posm[s->cur] + h.d.poff, 16, particles, 32, 16, n,
-                           hipMemcpyHostToDevice, s->stream),
-          "upload positions");
-    check(hipMemcpy2DAsync(s->velp[s->cur] + h.d.poff, 16, particles + 4, 32, 16, n,
-                           hipMemcpyHostToDevice, s->stream),
-          "upload velocities");
-  }
+  check(gs_upload_records(s, slot, particles, (uint32_t)n), "upload particles");
   h.d.n = (uint32_t)n;
   if (!h.fixed_max_nodes)  // new MortonTree(N): maxNodes = 2N + 4096 (mortonTree.js:88)
     h.d.max_nodes = std::min<uint64_t>(2 * n + 4096, h.d.ncap);
@@ -509,12 +502,13 @@ int gs_create(const gs_config* cfg, gs_sim** out) {
   if (cfg->n == 0 || cfg->n > GS_MAX_N) return GS_E_INVALID;
   const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
   if (!valid_wg(wg)) return GS_E_INVALID;
-  if (cfg->radix_bits != 0 && cfg->radix_bits != GS_RADIX_BITS) return GS_E_INVALID;
+  if (cfg->radix_bits != 0 && cfg->radix_bits != 8 && cfg->radix_bits != 10) return GS_E_INVALID;
   const uint32_t bucket = cfg->bucket ? cfg->bucket : 16;
   gs_sim* s = nullptr;
   try {
     if (cfg->max_nodes >= (1ull << 31)) return GS_E_INVALID;
-    s = make_pool(&cfg->n, &bucket, 1, cfg->flags, cfg->device, wg, cfg->max_nodes);
+    s = make_pool(&cfg->n, &bucket, 1, cfg->flags, cfg->device, wg,
+                  cfg->radix_bits ? cfg->radix_bits : 8, cfg->max_nodes);
   } catch (...) {
     s = nullptr;
   }
@@ -538,9 +532,11 @@ int gs_create_pool(const gs_pool_config* cfg, gs_sim** out) {
     if (cfg->capacity[k] == 0 || cfg->capacity[k] > GS_MAX_N) return GS_E_INVALID;
   const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
   if (!valid_wg(wg)) return GS_E_INVALID;
+  if (cfg->radix_bits != 0 && cfg->radix_bits != 8 && cfg->radix_bits != 10) return GS_E_INVALID;
   gs_sim* s = nullptr;
   try {
-    s = make_pool(cfg->capacity, cfg->bucket, cfg->nslots, cfg->flags, cfg->device, wg, 0);
+    s = make_pool(cfg->capacity, cfg->bucket, cfg->nslots, cfg->flags, cfg->device, wg,
+                  cfg->radix_bits ? cfg->radix_bits : 8, 0);
   } catch (...) {
     s = nullptr;
   }
@@ -566,7 +562,7 @@ int gs_pool_info(gs_sim* s, uint32_t* nslots, uint64_t* device_bytes) {
       const uint64_t P = s->ptot;
       *device_bytes = 2 * P * 32 + 4 * P * 4 + P * 4 + P * (4 + 16 + 4 + 4) +
                       (uint64_t)s->ntot * sizeof(NodeRec) + 2ull * s->build_words * 8 +
-                      (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX * 4 +
+                      (uint64_t)GS_HIST_WORDS * s->sort_tiles * 4 +
                       (uint64_t)s->walk_blocks * GS_NRED * 8;
     }
   });
@@ -780,12 +776,7 @@ int gs_slot_download_particles(gs_sim* s, uint32_t slot, float* dst, uint64_t n)
     const HostSlot& h = slot_of(s, slot);
     need(dst != nullptr && n == h.d.n, GS_E_INVALID, "download needs exactly n records");
     if (!n) return;
-    check(hipMemcpy2DAsync(dst, 32, s->posm[s->cur] + h.d.poff, 16, 16, n, hipMemcpyDeviceToHost,
-                           s->stream),
-          "download positions");
-    check(hipMemcpy2DAsync(dst + 4, 32, s->velp[s->cur] + h.d.poff, 16, 16, n,
-                           hipMemcpyDeviceToHost, s->stream),
-          "download velocities");
+    check(gs_download_records(s, slot, dst, (uint32_t)n), "download particles");
     sync(s);
   });
 }
@@ -795,12 +786,7 @@ int gs_read_positions(gs_sim* s, float* dst4, uint64_t n) {
     const HostSlot& h = s->hs[0];
     need(dst4 != nullptr && n == h.d.n, GS_E_INVALID, "read_positions needs N");
     // compact_positions (nbody_bh.wgsl:186-195): xyz of posm, w = energy
-    check(hipMemcpyAsync(dst4, s->posm[s->cur] + h.d.poff, 16 * n, hipMemcpyDeviceToHost,
-                         s->stream),
-          "positions");
-    check(hipMemcpy2DAsync(dst4 + 3, 16, s->energy + h.d.poff, 4, 4, n, hipMemcpyDeviceToHost,
-                           s->stream),
-          "energy");
+    check(gs_download_positions(s, 0, dst4, (uint32_t)n), "positions");
     sync(s);
   });
 }
@@ -810,7 +796,7 @@ int gs_download_codes(gs_sim* s, uint32_t* dst, uint64_t n, int sorted) {
     const HostSlot& h = s->hs[0];
     need(dst != nullptr && n == h.d.n, GS_E_INVALID, "codes need N");
     need(sorted ? h.have_tree : h.have_codes, GS_E_STATE, "no codes yet");
-    download_slot(s, 0, dst, sorted ? s->keys[0] : s->codes_raw, 4, n, "codes");
+    download_slot(s, 0, dst, sorted ? s->keys[gs_sorted_buf(s)] : s->codes_raw, 4, n, "codes");
   });
 }
 
@@ -823,7 +809,7 @@ int gs_slot_download_perm(gs_sim* s, uint32_t slot, uint32_t* dst, uint64_t n) {
     const HostSlot& h = slot_of(s, slot);
     need(dst != nullptr && n == h.d.n, GS_E_INVALID, "perm needs n");
     need(h.have_tree, GS_E_STATE, "no sort yet");
-    download_slot(s, slot, dst, s->vals[0], 4, n, "perm");
+    download_slot(s, slot, dst, s->vals[gs_sorted_buf(s)], 4, n, "perm");
   });
 }
 
@@ -988,7 +974,8 @@ int gs_radix_sort_indices(gs_sim* s, const uint32_t* codes, uint32_t n, uint32_t
     check(gs_launch_reset(s), "reset");
     check(gs_launch_hist(s), "hist");
     check(gs_launch_sort(s, false), "sort");
-    check(hipMemcpyAsync(perm, s->vals[0] + h.d.poff, 4ull * n, hipMemcpyDeviceToHost, s->stream),
+    check(hipMemcpyAsync(perm, s->vals[gs_sorted_buf(s)] + h.d.poff, 4ull * n,
+                         hipMemcpyDeviceToHost, s->stream),
           "perm");
     sync(s);
     h.have_tree = false;
@@ -1015,8 +1002,8 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     check(hipMemcpyAsync(s->posm[s->cur] + h.d.poff, pm.data(), 16ull * n, hipMemcpyHostToDevice,
                          s->stream),
           "pos");
-    check(hipMemcpyAsync(s->keys[0] + h.d.poff, sorted_codes, 4ull * n, hipMemcpyHostToDevice,
-                         s->stream),
+    check(hipMemcpyAsync(s->keys[gs_sorted_buf(s)] + h.d.poff, sorted_codes, 4ull * n,
+                         hipMemcpyHostToDevice, s->stream),
           "codes");
     // root segment and per-level sizes from the caller's rootSize
     DevScalars sc = read_scalars(s, 0);

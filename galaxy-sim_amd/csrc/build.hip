@@ -236,7 +236,8 @@ hipError_t gs_launch_build(gs_sim* s) {
   if (grid < 1) grid = 1;
   for (int d = 0; d <= GS_MAX_LEVEL; d++) {
     build_level_kernel<<<grid, GS_BUILD_TILE, 0, s->stream>>>(
-        s->keys[0], s->nodes, s->slots, s->sc, s->batch, s->nslots, d, s->build_status[d & 1],
+        s->keys[gs_sorted_buf(s)], s->nodes, s->slots, s->sc, s->batch, s->nslots, d,
+        s->build_status[d & 1],
         s->build_status[(d + 1) & 1], s->build_words);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -61,9 +61,8 @@ __global__ __launch_bounds__(256) void reset_kernel(DevBatch* batch, uint32_t* s
   clear_step_state(batch, sort_status, status_words);
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t per = (uint64_t)GS_SORT_PASSES * GS_RADIX;
-  for (uint64_t k = g; k < (uint64_t)nslots * per; k += stride)
-    (&sc[k / per].sort_hist[0][0])[k % per] = 0;
+  const uint64_t per = GS_HIST_WORDS;
+  for (uint64_t k = g; k < (uint64_t)nslots * per; k += stride) sc[k / per].sort_hist[k % per] = 0;
 }
 
 __global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ posm,
@@ -118,8 +117,7 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   const uint32_t slot = blockIdx.x;
   const DevSlot sl = slots[slot];
   DevScalars* sc = scs + slot;
-  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x)
-    (&sc->sort_hist[0][0])[k] = 0;
+  for (int k = threadIdx.x; k < GS_HIST_WORDS; k += blockDim.x) sc->sort_hist[k] = 0;
   float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   uint32_t c = 0;
   for (uint32_t b = threadIdx.x; b < sl.nchunks; b += blockDim.x) {
@@ -197,15 +195,25 @@ __device__ __forceinline__ uint32_t quant(float p, uint64_t mn, uint64_t s) {
   return (uint32_t)(v < 1023 ? v : 1023);  // part1by2 keeps the low 10 bits
 }
 
-__device__ __forceinline__ void flush_hist(uint32_t (*hist)[GS_RADIX], DevScalars* sc) {
+// digit histograms of one key for every pass, into LDS hist[pass * radix + digit]
+template <int BITS>
+__device__ __forceinline__ void count_digits(uint32_t* hist, uint32_t code) {
+  constexpr uint32_t RADIX = 1u << BITS;
+#pragma unroll
+  for (int ps = 0; ps < gs_sort_passes(BITS); ps++)
+    atomicAdd(&hist[ps * RADIX + ((code >> (ps * BITS)) & (RADIX - 1))], 1u);
+}
+
+template <int BITS>
+__device__ __forceinline__ void flush_hist(const uint32_t* hist, DevScalars* sc) {
   __syncthreads();
-  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) {
-    const uint32_t v = (&hist[0][0])[k];
-    if (v) atomicAdd(&(&sc->sort_hist[0][0])[k], v);
+  for (int k = threadIdx.x; k < gs_sort_passes(BITS) << BITS; k += blockDim.x) {
+    const uint32_t v = hist[k];
+    if (v) atomicAdd(&sc->sort_hist[k], v);
   }
 }
 
-template <bool KEEP_RAW>
+template <int BITS, bool KEEP_RAW>
 __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ posm,
                                                      const DevSlot* __restrict__ slots,
                                                      const uint2* __restrict__ chunk_tab,
@@ -217,8 +225,9 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
   const DevSlot sl = slots[ct.x];
   if (ct.y >= sl.n) return;  // the whole chunk is past the slot's particles
   DevScalars* sc = scs + ct.x;
-  __shared__ uint32_t hist[GS_SORT_PASSES][GS_RADIX];
-  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) (&hist[0][0])[k] = 0;
+  constexpr int HW = gs_sort_passes(BITS) << BITS;
+  __shared__ uint32_t hist[HW];
+  for (int k = threadIdx.x; k < HW; k += blockDim.x) hist[k] = 0;
   __syncthreads();
   const uint64_t mx = sc->min_bits[0], my = sc->min_bits[1], mz = sc->min_bits[2];
   const uint64_t s = sc->s_bits;
@@ -236,32 +245,30 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
     }
     keys[sl.poff + i] = code;
     vals[sl.poff + i] = i;
-#pragma unroll
-    for (int ps = 0; ps < GS_SORT_PASSES; ps++)
-      atomicAdd(&hist[ps][(code >> (ps * GS_RADIX_BITS)) & (GS_RADIX - 1)], 1u);
+    count_digits<BITS>(hist, code);
   }
-  flush_hist(hist, sc);
+  flush_hist<BITS>(hist, sc);
 }
 
 // digit histograms of host-supplied keys in slot 0 (gs_radix_sort_indices)
+template <int BITS>
 __global__ __launch_bounds__(256) void hist_kernel(uint32_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals,
                                                    const DevSlot* __restrict__ slots,
                                                    DevScalars* scs) {
   const DevSlot sl = slots[0];
-  __shared__ uint32_t hist[GS_SORT_PASSES][GS_RADIX];
-  for (int k = threadIdx.x; k < GS_SORT_PASSES * GS_RADIX; k += blockDim.x) (&hist[0][0])[k] = 0;
+  constexpr int HW = gs_sort_passes(BITS) << BITS;
+  __shared__ uint32_t hist[HW];
+  for (int k = threadIdx.x; k < HW; k += blockDim.x) hist[k] = 0;
   __syncthreads();
   const uint32_t end = min(((sl.n + GS_CHUNK - 1) / GS_CHUNK) * GS_CHUNK, sl.cap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
     if (i >= sl.n) keys[sl.poff + i] = 0xffffffffu;
     const uint32_t code = keys[sl.poff + i];
     vals[sl.poff + i] = i;
-#pragma unroll
-    for (int ps = 0; ps < GS_SORT_PASSES; ps++)
-      atomicAdd(&hist[ps][(code >> (ps * GS_RADIX_BITS)) & (GS_RADIX - 1)], 1u);
+    count_digits<BITS>(hist, code);
   }
-  flush_hist(hist, scs);
+  flush_hist<BITS>(hist, scs);
 }
 
 __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
@@ -276,8 +283,16 @@ inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
   return (uint32_t)(g < cap ? g : cap);
 }
 
-inline uint64_t status_words(const gs_sim* s) {
-  return (uint64_t)GS_SORT_PASSES * s->sort_tiles * GS_RADIX;
+inline uint64_t status_words(const gs_sim* s) { return gs_sort_status_words(s); }
+
+template <int BITS>
+void launch_encode(gs_sim* s, const float4* posm, bool keep_raw) {
+  if (keep_raw)
+    encode_kernel<BITS, true><<<s->chunks, 256, 0, s->stream>>>(
+        posm, s->slots, s->chunk_tab, s->keys[0], s->vals[0], s->codes_raw, s->sc);
+  else
+    encode_kernel<BITS, false><<<s->chunks, 256, 0, s->stream>>>(
+        posm, s->slots, s->chunk_tab, s->keys[0], s->vals[0], s->codes_raw, s->sc);
 }
 
 }  // namespace
@@ -300,19 +315,19 @@ hipError_t gs_launch_root(gs_sim* s) {
 }
 
 hipError_t gs_launch_encode(gs_sim* s, const float4* posm, bool keep_raw) {
-  if (keep_raw)
-    encode_kernel<true><<<s->chunks, 256, 0, s->stream>>>(posm, s->slots, s->chunk_tab, s->keys[0],
-                                                           s->vals[0], s->codes_raw, s->sc);
+  if (s->radix_bits == 10)
+    launch_encode<10>(s, posm, keep_raw);
   else
-    encode_kernel<false><<<s->chunks, 256, 0, s->stream>>>(posm, s->slots, s->chunk_tab,
-                                                            s->keys[0], s->vals[0], s->codes_raw,
-                                                            s->sc);
+    launch_encode<8>(s, posm, keep_raw);
   return hipGetLastError();
 }
 
 hipError_t gs_launch_hist(gs_sim* s) {
-  hist_kernel<<<grid_for(s->hs[0].d.cap, 256, 4096), 256, 0, s->stream>>>(s->keys[0], s->vals[0],
-                                                                            s->slots, s->sc);
+  const uint32_t g = grid_for(s->hs[0].d.cap, 256, 4096);
+  if (s->radix_bits == 10)
+    hist_kernel<10><<<g, 256, 0, s->stream>>>(s->keys[0], s->vals[0], s->slots, s->sc);
+  else
+    hist_kernel<8><<<g, 256, 0, s->stream>>>(s->keys[0], s->vals[0], s->slots, s->sc);
   return hipGetLastError();
 }
 

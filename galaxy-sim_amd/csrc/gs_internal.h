@@ -64,10 +64,10 @@ struct DevParams {
   float4 halo_c1, halo_c2;
 };
 
-// Sort digit width and tiling.
-#define GS_RADIX_BITS 8
-#define GS_RADIX 256
-#define GS_SORT_PASSES 4
+// Sort digit widths (8 -> 4 passes, 10 -> 3 passes over the 30 code bits)
+// and tiling.
+#define GS_HIST_WORDS 3072  // max over widths of passes x radix (4 x 256, 3 x 1024)
+__host__ __device__ constexpr int gs_sort_passes(int bits) { return (30 + bits - 1) / bits; }
 #define GS_SORT_BLOCK 256
 #define GS_SORT_ITEMS 16
 #define GS_SORT_TILE (GS_SORT_BLOCK * GS_SORT_ITEMS)
@@ -104,7 +104,7 @@ struct DevScalars {
   uint32_t level_start[GS_MAX_LEVELS + 1];
   uint32_t level_count[GS_MAX_LEVELS + 1];
   float level_size[GS_MAX_LEVELS + 1];
-  uint32_t sort_hist[GS_SORT_PASSES][GS_RADIX];  // this slot's digit histograms
+  uint32_t sort_hist[GS_HIST_WORDS];  // this slot's digit histograms [pass][digit]
 };
 
 // Pool-wide bookkeeping.
@@ -163,6 +163,9 @@ __device__ __forceinline__ uint32_t gs_find_slot(const uint32_t* s_pre, uint32_t
   return lo;
 }
 
+// ping-pong buffer that holds the sorted (key, index) pairs after a sort
+inline int gs_sorted_buf(const gs_sim* s);
+
 struct StageTimer {
   int stage;
   hipEvent_t a, b;
@@ -180,6 +183,7 @@ struct gs_sim {
   int device = 0;
   hipStream_t stream = nullptr;
   uint32_t flags = 0, walk_wg = 256;
+  uint32_t radix_bits = 8;  // sort digit width: 8 or 10
   uint32_t nslots = 1;
   uint64_t ptot = 0;         // sum of slot capacities (aligned)
   uint64_t ntot = 0;         // sum of node pools
@@ -195,7 +199,7 @@ struct gs_sim {
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* codes_raw = nullptr;
-  uint32_t* sort_status = nullptr;  // GS_SORT_PASSES x sort_tiles x radix
+  uint32_t* sort_status = nullptr;  // passes x sort_tiles x radix (<= GS_HIST_WORDS x tiles)
   uint64_t* build_status[2] = {nullptr, nullptr};
   float* bbox_part = nullptr;       // chunks x 8
   NodeRec* nodes = nullptr;         // ntot records
@@ -228,6 +232,11 @@ struct gs_sim {
   std::string err;
 };
 
+inline int gs_sorted_buf(const gs_sim* s) { return gs_sort_passes((int)s->radix_bits) & 1; }
+inline uint64_t gs_sort_status_words(const gs_sim* s) {
+  return (uint64_t)gs_sort_passes((int)s->radix_bits) * s->sort_tiles * (1u << s->radix_bits);
+}
+
 // ---- launchers (stream-ordered; return hipError_t of the launch) ----------
 hipError_t gs_launch_bbox(gs_sim* s, const float4* posm);
 hipError_t gs_launch_root(gs_sim* s);
@@ -244,3 +253,8 @@ hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, c
                               uint32_t num);
 hipError_t gs_launch_gather(gs_sim* s, uint32_t slot, const uint32_t* perm);
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
+// 32-byte host records <-> (posm, velp) of one slot, staged contiguously in
+// the slot's range of the non-current ping-pong buffers (stream-ordered)
+hipError_t gs_upload_records(gs_sim* s, uint32_t slot, const float* host, uint32_t n);
+hipError_t gs_download_records(gs_sim* s, uint32_t slot, float* host, uint32_t n);
+hipError_t gs_download_positions(gs_sim* s, uint32_t slot, float* host4, uint32_t n);

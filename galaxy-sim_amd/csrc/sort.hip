@@ -1,16 +1,20 @@
 // Stable LSD radix sort of (Morton code, particle index) pairs — the device
 // form of radixSortIndices (mortonTree.js:68-84).  A stable sort by key of the
-// identity index array is unique, so 4 passes of 8-bit digits give exactly the
-// permutation of the reference's 3 passes of 10-bit digits.
+// identity index array is unique, so 4 passes of 8-bit digits or 3 passes of
+// 10-bit digits give exactly the permutation of the reference's 3 passes of
+// 10-bit digits.  (10-bit passes look at bits 0..29 only; padding keys
+// 0xffffffff then tie with code 0x3fffffff and stay behind it, because their
+// indices are larger and the sort is stable.)
 //
 // One kernel per pass ("onesweep") over every slot of the pool; each slot is
 // sorted on its own (slot-local keys, values and output positions):
 //   1. a block takes the next 4096-key tile in launch order (atomic ticket)
 //      and maps it to (slot, tile of the slot) through a static table; tiles
 //      past the slot's active particles exit;
-//   2. each wavefront ranks its 16 x 64 keys in order: per 64-key round, eight
-//      wave ballots give every lane the mask of lanes holding the same digit
-//      (peers), rank = running per-wave digit count + popcount(peers below);
+//   2. each wavefront ranks its 16 x 64 keys in order: per 64-key round, one
+//      wave ballot per digit bit gives every lane the mask of lanes holding
+//      the same digit (peers), rank = running per-wave digit count +
+//      popcount(peers below);
 //   3. per digit, the tile publishes its count and looks back over earlier
 //      tiles of its slot (decoupled look-back; a slot's first tile starts the
 //      chain) to get its output offset, added to the digit's base from the
@@ -22,7 +26,8 @@
 // other ping-pong buffer without a separate pass over the index array.
 //
 // HBM per pass: 8 B read + 8 B written per key (+ 32 B gathered + 32 B
-// written per particle in the last pass).
+// written per particle in the last pass).  8-bit digits write 256 runs of
+// ~16 keys per tile; 10-bit digits save a pass but write 1024 runs of ~4.
 #include "gs_internal.h"
 
 namespace {
@@ -55,7 +60,7 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w
   return off + x - v;
 }
 
-template <bool GATHER>
+template <int BITS, bool GATHER>
 __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int pass, uint32_t* status,
@@ -63,18 +68,21 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     const uint32_t* __restrict__ tile_tab, const float4* __restrict__ pin,
     const float4* __restrict__ qin, float4* __restrict__ pout, float4* __restrict__ qout) {
   constexpr int ITEMS = GS_SORT_ITEMS;
+  constexpr uint32_t RADIX = 1u << BITS;
+  constexpr int DPT = (int)RADIX / GS_SORT_BLOCK;  // digits per thread
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_wh[GS_SORT_BLOCK / 64][GS_RADIX];
-  __shared__ uint32_t s_dstart[GS_RADIX];
-  __shared__ uint32_t s_gbase[GS_RADIX];
+  __shared__ uint32_t s_wh[GS_SORT_BLOCK / 64][RADIX];
+  __shared__ uint32_t s_dstart[RADIX];
+  __shared__ uint32_t s_gbase[RADIX];
   __shared__ uint32_t s_wsum[GS_SORT_BLOCK / 64];
   __shared__ uint32_t s_k[GS_SORT_TILE];
   __shared__ uint32_t s_v[GS_SORT_TILE];
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int shift = pass * GS_RADIX_BITS;
+  const int shift = pass * BITS;
   if (tid == 0) s_tile = atomicAdd(&batch->tickets[pass], 1u);
-  for (int k = tid; k < (GS_SORT_BLOCK / 64) * GS_RADIX; k += GS_SORT_BLOCK) (&s_wh[0][0])[k] = 0;
+  for (uint32_t k = tid; k < (GS_SORT_BLOCK / 64) * RADIX; k += GS_SORT_BLOCK)
+    (&s_wh[0][0])[k] = 0;
   __syncthreads();
   const uint32_t gtile = s_tile;
   const uint32_t tt = tile_tab[gtile];
@@ -83,14 +91,13 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
   const uint32_t active = (sl.n + GS_SORT_TILE - 1) / GS_SORT_TILE;
   if (tile >= active) return;  // no later tile of this slot is active either
   // keys exist up to the end of the slot's last active tile (inside the cap)
-  const uint32_t nk = min(active * (uint32_t)GS_SORT_TILE, sl.cap);
+  const uint32_t n = min(active * (uint32_t)GS_SORT_TILE, sl.cap);
   kin += sl.poff;
   vin += sl.poff;
   kout += sl.poff;
   vout += sl.poff;
   const uint32_t base = tile * GS_SORT_TILE;
   const uint64_t below_mask = (1ull << lane) - 1;
-  const uint32_t n = nk;
 
   uint32_t key[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -99,10 +106,10 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     const bool valid = idx < n;
     const uint32_t k = valid ? kin[idx] : 0u;
     key[r] = k;
-    const uint32_t d = (k >> shift) & (GS_RADIX - 1);
+    const uint32_t d = (k >> shift) & (RADIX - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < GS_RADIX_BITS; b++) {
+    for (int b = 0; b < BITS; b++) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
@@ -115,52 +122,66 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
   }
   __syncthreads();
 
-  // per digit (one per thread): wave-exclusive offsets and the tile count
-  const uint32_t d = tid;
-  uint32_t c[GS_SORT_BLOCK / 64];
-  uint32_t total = 0;
+  // digits tid*DPT .. tid*DPT+DPT-1: wave-exclusive offsets and tile counts
+  uint32_t total[DPT], hist[DPT];
+  uint32_t tsum = 0, hsum = 0;
+  uint32_t* st[DPT];
 #pragma unroll
-  for (int k = 0; k < GS_SORT_BLOCK / 64; k++) {
-    c[k] = s_wh[k][d];
-    s_wh[k][d] = total;
-    total += c[k];
-  }
-  uint32_t* st = status + (size_t)gtile * GS_RADIX + d;
-  st_agent(st, (tile == 0 ? kPre : kAgg) | total);
-
-  const uint32_t hist_excl = block_excl_scan256(scs[slot].sort_hist[pass][d], s_wsum);
-  const uint32_t tile_excl = block_excl_scan256(total, s_wsum);
-
-  uint32_t excl = 0;
-  if (tile > 0) {
-    int32_t j = (int32_t)gtile - 1;  // a slot's tiles are consecutive
-    uint32_t spins = 0;
-    for (;;) {
-      const uint32_t v = ld_agent(status + (size_t)j * GS_RADIX + d);
-      const uint32_t f = v & ~kVal;
-      if (f == 0) {
-        if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
-          atomicOr(&batch->status, GS_ST_SPIN);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      excl += v & kVal;
-      if (f == kPre) break;
-      j--;
+  for (int j = 0; j < DPT; j++) {
+    const uint32_t d = (uint32_t)tid * DPT + j;
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < GS_SORT_BLOCK / 64; k++) {
+      const uint32_t c = s_wh[k][d];
+      s_wh[k][d] = t;
+      t += c;
     }
-    st_agent(st, kPre | (excl + total));
+    total[j] = t;
+    st[j] = status + (size_t)gtile * RADIX + d;
+    st_agent(st[j], (tile == 0 ? kPre : kAgg) | t);
+    hist[j] = scs[slot].sort_hist[pass * RADIX + d];
+    tsum += t;
+    hsum += hist[j];
   }
-  s_gbase[d] = hist_excl + excl;
-  s_dstart[d] = tile_excl;
+  uint32_t hist_excl = block_excl_scan256(hsum, s_wsum);
+  uint32_t tile_excl = block_excl_scan256(tsum, s_wsum);
+
+#pragma unroll
+  for (int j = 0; j < DPT; j++) {
+    const uint32_t d = (uint32_t)tid * DPT + j;
+    uint32_t excl = 0;
+    if (tile > 0) {
+      int32_t jt = (int32_t)gtile - 1;  // a slot's tiles are consecutive
+      uint32_t spins = 0;
+      for (;;) {
+        const uint32_t v = ld_agent(status + (size_t)jt * RADIX + d);
+        const uint32_t f = v & ~kVal;
+        if (f == 0) {
+          if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
+            atomicOr(&batch->status, GS_ST_SPIN);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += v & kVal;
+        if (f == kPre) break;
+        jt--;
+      }
+      st_agent(st[j], kPre | (excl + total[j]));
+    }
+    s_gbase[d] = hist_excl + excl;
+    s_dstart[d] = tile_excl;
+    hist_excl += hist[j];
+    tile_excl += total[j];
+  }
   __syncthreads();
 
 #pragma unroll
   for (int r = 0; r < ITEMS; r++) {
     const uint32_t idx = base + (uint32_t)w * (64 * ITEMS) + (uint32_t)r * 64 + lane;
     if (idx < n) {
-      const uint32_t dd = (key[r] >> shift) & (GS_RADIX - 1);
+      const uint32_t dd = (key[r] >> shift) & (RADIX - 1);
       const uint32_t pos = s_dstart[dd] + s_wh[w][dd] + rank[r];
       s_k[pos] = key[r];
       s_v[pos] = vin[idx];
@@ -171,7 +192,7 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
   const uint32_t tile_n = n - base < (uint32_t)GS_SORT_TILE ? n - base : (uint32_t)GS_SORT_TILE;
   for (uint32_t i = tid; i < tile_n; i += GS_SORT_BLOCK) {
     const uint32_t k = s_k[i];
-    const uint32_t dd = (k >> shift) & (GS_RADIX - 1);
+    const uint32_t dd = (k >> shift) & (RADIX - 1);
     const uint32_t out = s_gbase[dd] + (i - s_dstart[dd]);
     const uint32_t v = s_v[i];
     if (out < n) {
@@ -185,23 +206,29 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
   }
 }
 
-}  // namespace
-
-hipError_t gs_launch_sort(gs_sim* s, bool gather) {
-  for (int p = 0; p < GS_SORT_PASSES; p++) {
-    uint32_t* status = s->sort_status + (size_t)p * s->sort_tiles * GS_RADIX;
+template <int BITS>
+hipError_t launch_sort(gs_sim* s, bool gather) {
+  constexpr int RADIX = 1 << BITS, PASSES = gs_sort_passes(BITS);
+  for (int p = 0; p < PASSES; p++) {
+    uint32_t* status = s->sort_status + (size_t)p * s->sort_tiles * RADIX;
     const int a = p & 1, b = a ^ 1;
-    if (gather && p == GS_SORT_PASSES - 1)
-      onesweep_kernel<true><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
+    if (gather && p == PASSES - 1)
+      onesweep_kernel<BITS, true><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
           s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
           s->tile_tab, s->posm[s->cur], s->velp[s->cur], s->posm[s->cur ^ 1],
           s->velp[s->cur ^ 1]);
     else
-      onesweep_kernel<false><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
+      onesweep_kernel<BITS, false><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
           s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
           s->tile_tab, nullptr, nullptr, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t gs_launch_sort(gs_sim* s, bool gather) {
+  return s->radix_bits == 10 ? launch_sort<10>(s, gather) : launch_sort<8>(s, gather);
 }

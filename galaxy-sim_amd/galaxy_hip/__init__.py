@@ -52,7 +52,7 @@ class PoolConfig(ctypes.Structure):
     _fields_ = [("nslots", ctypes.c_uint32), ("capacity", ctypes.c_void_p),
                 ("bucket", ctypes.c_void_p), ("flags", ctypes.c_uint32),
                 ("device", ctypes.c_int32), ("walk_wg", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 5)]
+                ("radix_bits", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
 
 
 class Params(ctypes.Structure):
@@ -169,12 +169,13 @@ class NBodyGPU:
     """
 
     def __init__(self, particles, bucket=16, max_nodes=0, exact_force=False, timing=False,
-                 device=-1, walk_wg=0):
+                 device=-1, walk_wg=0, radix_bits=0, graph=True):
         particles = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 8)
         self.N = particles.shape[0]
         cfg = Config(n=self.N, bucket=bucket, max_nodes=max_nodes,
                      flags=(FLAG_EXACT_FORCE if exact_force else 0) |
-                     (FLAG_TIMING if timing else 0), device=device, walk_wg=walk_wg)
+                     (FLAG_TIMING if timing else 0) | (0 if graph else FLAG_NO_GRAPH),
+                     device=device, walk_wg=walk_wg, radix_bits=radix_bits)
         h = ctypes.c_void_p()
         rc = lib().gs_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != GS_OK:
@@ -327,7 +328,7 @@ class GalaxyPool:
     """
 
     def __init__(self, capacities, buckets=None, exact_force=False, timing=False, device=-1,
-                 walk_wg=0, graph=True):
+                 walk_wg=0, graph=True, radix_bits=0):
         caps = np.ascontiguousarray(capacities, dtype=np.uint32)
         self.nslots = int(caps.size)
         self.capacity = caps.copy()
@@ -337,7 +338,7 @@ class GalaxyPool:
                          bucket=bk.ctypes.data if bk is not None else None,
                          flags=(FLAG_EXACT_FORCE if exact_force else 0) |
                          (FLAG_TIMING if timing else 0) | (0 if graph else FLAG_NO_GRAPH),
-                         device=device, walk_wg=walk_wg)
+                         device=device, walk_wg=walk_wg, radix_bits=radix_bits)
         h = ctypes.c_void_p()
         rc = lib().gs_create_pool(ctypes.byref(cfg), ctypes.byref(h))
         if rc != GS_OK:
@@ -453,13 +454,13 @@ class GalaxyPool:
 _scratch = {}
 
 
-def _scratch_sim(n):
+def _scratch_sim(n, radix_bits=0):
     """A resident handle sized for n particles, reused across calls."""
-    sim = _scratch.get(n)
+    sim = _scratch.get((n, radix_bits))
     if sim is None:
-        sim = NBodyGPU(np.zeros((n, 8), np.float32))
+        sim = NBodyGPU(np.zeros((n, 8), np.float32), radix_bits=radix_bits)
         _scratch.clear()
-        _scratch[n] = sim
+        _scratch[(n, radix_bits)] = sim
     return sim
 
 
@@ -473,12 +474,13 @@ def compute_morton_codes(pos, N, stride, codes=None):
     return out, rs.value
 
 
-def radix_sort_indices(codes, N=None):
-    """radixSortIndices (mortonTree.js:68-84): the stable sorted index array."""
+def radix_sort_indices(codes, N=None, radix_bits=0):
+    """radixSortIndices (mortonTree.js:68-84): the stable sorted index array
+    (codes < 2^30; radix_bits 8 or 10 selects the device digit width)."""
     codes = np.ascontiguousarray(codes, dtype=np.uint32)
     N = codes.size if N is None else N
     out = np.empty(N, np.uint32)
-    _scratch_sim(N)._call("gs_radix_sort_indices", _ptr(codes), N, _ptr(out))
+    _scratch_sim(N, radix_bits)._call("gs_radix_sort_indices", _ptr(codes), N, _ptr(out))
     return out
 
 

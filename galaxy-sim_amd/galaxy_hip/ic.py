@@ -208,12 +208,53 @@ def batch_sizes(count=64, lo=4096, hi=262144, seed=SEED_BASE + 4):
     return np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
 
 
+def batch64(count=64, lo=4096, hi=262144, seed=SEED_BASE + 4):
+    """The batched workload: `count` galaxies, N log-uniform in [lo, hi],
+    each a disk or a Plummer sphere (a = 1) with probability 1/2.
+    Returns a list of (preset, records)."""
+    sizes = batch_sizes(count, lo, hi, seed)
+    kinds = Stream(seed ^ 0x9E5E7).uniform(count) < 0.5
+    out = []
+    for k, (n, is_disk) in enumerate(zip(sizes.tolist(), kinds.tolist())):
+        s = seed + 0x1000 * (k + 1)
+        out.append(("disk", disk(n, seed=s)) if is_disk else ("plummer", plummer(n, seed=s)))
+    return out
+
+
+def serve_jobs(count, rate=20.0, lo=8192, hi=1048576, steps=200, seed=SEED_BASE + 5):
+    """The served workload's job stream: Poisson arrivals (mean `rate` jobs/s,
+    exponential gaps), N log-uniform in [lo, hi], preset uniform over disk /
+    Plummer sphere / two-disk collision.  Returns dicts with the arrival time
+    (s), n, preset, steps and a per-job seed; records come from job_records."""
+    st = Stream(seed ^ 0x5E77E)
+    gaps = -np.log(st.open_uniform(count)) / rate
+    t = np.cumsum(gaps) - gaps[0]
+    u = st.uniform(count)
+    sizes = np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+    kinds = np.minimum((st.uniform(count) * 3).astype(np.int64), 2)
+    names = ("disk", "plummer", "collision")
+    return [dict(id=k, arrival=float(t[k]), n=int(sizes[k]), preset=names[kinds[k]],
+                 steps=steps, seed=seed + 0x1000 * (k + 1)) for k in range(count)]
+
+
+def job_records(job):
+    n, s = job["n"], job["seed"]
+    if job["preset"] == "disk":
+        return disk(n, seed=s)
+    if job["preset"] == "plummer":
+        return plummer(n, seed=s)
+    return collision(n, seed=s)
+
+
 CONFIGS = {
     # name: (builder, params) — the workloads of the benchmark table
     "cpu_disk_16k": dict(n=16384, theta=0.7, epsilon=0.02, bucket=16, dt=0.005),
     "disk_500k": dict(n=500000, theta=0.7, epsilon=0.01, bucket=16, dt=0.005, steps=100,
                       gas_fraction=0.3, halo_mass=5.0, halo_a=3.0, gas_drag=0.25),
     "collision_4m": dict(n=4194304, theta=0.5, epsilon=0.005, bucket=32, dt=0.005),
+    "batch64": dict(count=64, lo=4096, hi=262144, theta=0.6, epsilon=0.01, dt=0.005),
+    "serve": dict(rate=20.0, lo=8192, hi=1048576, steps=200, theta=0.5, epsilon=0.01, dt=0.005,
+                  hbm_budget=200e9),
 }
 
 

@@ -66,7 +66,8 @@ typedef struct gs_config {
   int32_t device;     /* HIP device ordinal, -1 = current device */
   uint32_t walk_wg;   /* walk workgroup size, a power of two in [64, 1024];
                          0 -> tuned default */
-  uint32_t radix_bits; /* sort digit width: 0 -> 8 (only 8 is built) */
+  uint32_t radix_bits; /* sort digit width, 8 (4 passes) or 10 (3 passes);
+                          0 -> 8.  The permutation is the same. */
   uint32_t reserved[4];
 } gs_config;
 
@@ -202,7 +203,8 @@ typedef struct gs_pool_config {
   uint32_t flags;            /* GS_FLAG_* */
   int32_t device;            /* HIP device ordinal, -1 = current device */
   uint32_t walk_wg;          /* as gs_config.walk_wg */
-  uint32_t reserved[5];
+  uint32_t radix_bits;       /* as gs_config.radix_bits */
+  uint32_t reserved[4];
 } gs_pool_config;
 
 int gs_create_pool(const gs_pool_config* cfg, gs_sim** out);

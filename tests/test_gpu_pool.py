@@ -165,6 +165,27 @@ def test_pool_slot_reuse():
     assert np.array_equal(got_1.view(np.uint32), parts1.view(np.uint32))
 
 
+def test_pool_radix10_equals_radix8():
+    """The 10-bit-digit sort gives the same permutation, so whole steps agree."""
+    banner("pool radix 10 vs 8")
+    out = []
+    for radix in (8, 10):
+        pool = galaxy_hip.GalaxyPool([j[0] for j in JOBS], buckets=[j[3] for j in JOBS],
+                                     radix_bits=radix)
+        try:
+            for k, (cap, n, preset, bucket, extra) in enumerate(JOBS):
+                pool.upload(k, _galaxy(preset, n, k), **_params(extra))
+            pool.step(3, sort_every=1)
+            out.append([(pool.particles(k), pool.perm(k) if JOBS[k][1] else None)
+                        for k in range(len(JOBS))])
+        finally:
+            pool.destroy()
+    for (pa, qa), (pb, qb) in zip(*out):
+        assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32))
+        if qa is not None:
+            assert np.array_equal(qa, qb)
+
+
 def test_pool_rejects_bad_sizes():
     banner("pool argument checks")
     pool = galaxy_hip.GalaxyPool([1024, 2048])
@@ -178,3 +199,5 @@ def test_pool_rejects_bad_sizes():
         assert pool.info()["nslots"] == 2
     finally:
         pool.destroy()
+    with pytest.raises(galaxy_hip.GalaxyError):
+        galaxy_hip.GalaxyPool([1024], radix_bits=9)

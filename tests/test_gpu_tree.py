@@ -14,9 +14,10 @@ from oracle import oracle
 pytestmark = pytest.mark.gpu
 
 
-def _sorted_tree_case(case, data):
+def _sorted_tree_case(case, data, radix=8):
     n = case["N"]
-    sim = galaxy_hip.NBodyGPU(data.reshape(n, 8), bucket=16, max_nodes=case["maxNodes"])
+    sim = galaxy_hip.NBodyGPU(data.reshape(n, 8), bucket=16, max_nodes=case["maxNodes"],
+                              radix_bits=radix)
     try:
         sim.encode()
         codes = sim.codes(sorted=False)
@@ -53,11 +54,12 @@ def test_tree_matches_mortonTree_js_golden(case):
                              f"word {bad[0] % 16}: {nodes[bad[0]]:#x} vs {want[bad[0]]:#x}")
 
 
+@pytest.mark.parametrize("radix", [8, 10])
 @pytest.mark.parametrize("case", cases.cases("digest"), ids=lambda c: c["name"])
-def test_tree_matches_mortonTree_js_digests(case):
-    banner(f"digest {case['name']} N={case['N']}")
+def test_tree_matches_mortonTree_js_digests(case, radix):
+    banner(f"digest {case['name']} N={case['N']} radix={radix}")
     data = cases.input_of(case)
-    codes, perm, sorted_, scodes, info, nodes = _sorted_tree_case(case, data)
+    codes, perm, sorted_, scodes, info, nodes = _sorted_tree_case(case, data, radix)
     assert cases.sha(codes) == case["sha"]["codes"]
     assert cases.sha(perm) == case["sha"]["perm"]
     assert cases.sha(sorted_.reshape(-1)) == case["sha"]["sorted"]
@@ -75,26 +77,29 @@ def test_stride4_codes_host_surface():
     assert np.float64(size).tobytes().hex() == case["rootSizeHex"]
 
 
+@pytest.mark.parametrize("radix", [8, 10])
 @pytest.mark.parametrize("n", [1, 5, 4095, 4096, 4097, 100003, 1 << 20])
-def test_radix_sort_is_stable_sort(n):
-    banner(f"radix sort n={n}")
+def test_radix_sort_is_stable_sort(n, radix):
+    banner(f"radix sort n={n} radix={radix}")
     rng = np.random.default_rng(n)
     codes = rng.integers(0, 1 << 30, n, dtype=np.uint32)
     if n > 10:
         codes[: n // 4] = codes[n // 4: 2 * (n // 4)]
         codes[-(n // 8):] = 7  # long run of equal keys across tiles
-    perm = galaxy_hip.radix_sort_indices(codes)
+    perm = galaxy_hip.radix_sort_indices(codes, radix_bits=radix)
     assert np.array_equal(perm, np.argsort(codes, kind="stable").astype(np.uint32))
 
 
-@pytest.mark.parametrize("name,n,bucket", [("cpu_disk_16k", 16384, 16),
-                                           ("collision_4m", 200000, 32),
-                                           ("disk_500k", 500000, 16)])
-def test_tree_matches_oracle(name, n, bucket):
-    banner(f"oracle tree {name} n={n} bucket={bucket}")
+@pytest.mark.parametrize("name,n,bucket,radix", [("cpu_disk_16k", 16384, 16, 8),
+                                                 ("collision_4m", 200000, 32, 8),
+                                                 ("disk_500k", 500000, 16, 8),
+                                                 ("cpu_disk_16k", 16384, 16, 10),
+                                                 ("collision_4m", 200000, 32, 10)])
+def test_tree_matches_oracle(name, n, bucket, radix):
+    banner(f"oracle tree {name} n={n} bucket={bucket} radix={radix}")
     data = ic.make(name, n=n)
     ref = oracle_tree(data, bucket=bucket)
-    sim = galaxy_hip.NBodyGPU(data, bucket=bucket)
+    sim = galaxy_hip.NBodyGPU(data, bucket=bucket, radix_bits=radix)
     try:
         sim.sort()
         perm = sim.perm()
