@@ -101,7 +101,11 @@ def bench_batch(args):
     params = dict(dt=c["dt"], G=1.0, epsilon=c["epsilon"], theta=c["theta"])
     log(f"batch64: {len(datas)} galaxies, N total {total_n}, min {min(sizes)}, max {max(sizes)}")
     t0 = time.perf_counter()
-    tuned = autotune.tune(datas, params, log=log)
+    if args.tuned:
+        wg, bucket, radix = (int(x) for x in args.tuned.split(","))
+        tuned = {"walk_wg": wg, "bucket": bucket, "radix_bits": radix, "trials": []}
+    else:
+        tuned = autotune.tune(datas, params, log=log)
     t_tune = time.perf_counter() - t0
     wg, bucket, radix = tuned["walk_wg"], tuned["bucket"], tuned["radix_bits"]
     log(f"autotune: wg={wg} bucket={bucket} radix={radix} ({len(tuned['trials'])} trials, "
@@ -203,18 +207,21 @@ def bench_serve(args, tuned):
     caps = serve.slot_classes(c["lo"], c["hi"], args.serve_per_class)
     wg = tuned.get("walk_wg", 0)
     radix = tuned.get("radix_bits", 0)
+    bucket = tuned.get("bucket", 16)
     with serve.GalaxyServer(caps, hbm_budget=c["hbm_budget"],
-                            resident_limit=args.serve_resident, bucket=16, walk_wg=wg,
+                            resident_limit=args.serve_resident, bucket=bucket, walk_wg=wg,
                             radix_bits=radix, keep_results=False) as srv:
-        js = [serve.Job(j["id"], r, params, j["steps"], j["arrival"]) for j, r in zip(jobs, records)]
+        js = [serve.Job(j["id"], r, params, j["steps"], j["arrival"], bucket=bucket)
+              for j, r in zip(jobs, records)]
         summ = srv.run(js, realtime=True, log=log)
-    log(f"served: {summ['throughput'] / 1e6:.1f} M particle-steps/s, step p50 "
-        f"{summ['step_p50_ms']:.2f} ms p99 {summ['step_p99_ms']:.2f} ms")
+    log(f"served: {summ['throughput'] / 1e6:.1f} M particle-steps/s, job-step p50 "
+        f"{summ['step_p50_ms']:.2f} ms p99 {summ['step_p99_ms']:.2f} ms (server step p50 "
+        f"{summ['server_step_p50_ms']:.2f} p99 {summ['server_step_p99_ms']:.2f})")
     # plain: every job alone on its own handle, timed over a few steps
     t_plain = 0.0
     work = 0
     for j, r in zip(jobs, records):
-        sim = galaxy_hip.NBodyGPU(r, bucket=16, walk_wg=wg, radix_bits=radix)
+        sim = galaxy_hip.NBodyGPU(r, bucket=bucket, walk_wg=wg, radix_bits=radix)
         try:
             sim.update_params(**params)
             sim.step(1, sort_every=1)
@@ -229,9 +236,12 @@ def bench_serve(args, tuned):
     plain = work / t_plain
     log(f"served plain (one job at a time): {plain / 1e6:.1f} M particle-steps/s")
     summ.update({
-        "workload": "serve: Poisson arrivals 20 jobs/s, N log-uniform in [8192, 1048576], "
-                    f"{args.serve_steps} steps per job, disk / Plummer / collision uniform, "
-                    "theta=0.5, eps=0.01, dt=0.005, 200 GB HBM budget",
+        "workload": f"serve: {len(jobs)} jobs, Poisson arrivals 20 jobs/s, N log-uniform in "
+                    f"[8192, 1048576], {args.serve_steps} steps per job, disk / Plummer / "
+                    "collision uniform, theta=0.5, eps=0.01, dt=0.005, 200 GB HBM budget, "
+                    f"bucket {bucket}",
+        "latency_note": "step_p50/p99 count each server step once per job it advanced; "
+                        "server_step_p50/p99 count each server step once",
         "value": summ["throughput"], "unit": "particle-steps/s",
         "resident_limit": args.serve_resident,
         "plain_one_at_a_time": plain, "speedup_vs_plain": summ["throughput"] / plain,
@@ -240,7 +250,81 @@ def bench_serve(args, tuned):
     return summ
 
 
-def copy_probe(mib):
+COPY_PROBE_MIB = 256
+
+
+def probe_walk(args):
+    """Child-process workload of the counter passes: a 256 MiB device copy
+    (the calibration kernel of known size), then the headline galaxy sorted,
+    refreshed and stepped twice."""
+    copy_probe(COPY_PROBE_MIB, quiet=True)
+    import galaxy_hip
+    from galaxy_hip import ic
+    sim = galaxy_hip.NBodyGPU(ic.make("collision_4m", n=args.n), bucket=32)
+    sim.update_params(dt=0.005, G=1.0, epsilon=0.005, theta=0.5)
+    sim.step(2, sort_every=1)
+    sim.sync()
+    sim.destroy()
+
+
+def _pmc_pass(counter, args, workdir):
+    """One rocprofv3 counter pass over probe_walk; per-kernel mean counter
+    value per dispatch (KiB for FETCH_SIZE / WRITE_SIZE)."""
+    import shutil
+    import signal
+    import sqlite3
+    import subprocess
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    outdir = os.path.join(workdir, counter.lower())
+    cmd = [exe, "--pmc", counter, "--kernel-include-regex", "walk_kernel|copyBuffer",
+           "-d", outdir, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__),
+           "--probe-walk", "--n", str(args.n)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                         stderr=subprocess.DEVNULL, start_new_session=True)
+    try:
+        rc = p.wait(timeout=args.pmc_timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
+        raise RuntimeError(f"{counter} pass timed out")
+    dbs = [os.path.join(r, f) for r, _, fs in os.walk(outdir) for f in fs if f.endswith(".db")]
+    if rc != 0 and not dbs:
+        raise RuntimeError(f"{counter} pass exited {rc}")
+    vals = {}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        for name, value in con.execute(
+                "select kernel_name, value from counters_collection where counter_name = ?",
+                (counter,)):
+            key = "walk" if "walk_kernel" in name else "copy" if "copyBuffer" in name else None
+            if key:
+                vals.setdefault(key, []).append(float(value))
+        con.close()
+    if "walk" not in vals or "copy" not in vals:
+        raise RuntimeError(f"{counter} pass recorded no walk/copy dispatches")
+    # the calibration copy is the largest copy dispatch (host uploads are
+    # smaller blit copies of the same kernel)
+    return {"walk": float(np.mean(vals["walk"])), "copy": float(max(vals["copy"]))}
+
+
+def pmc_traffic(args):
+    """HBM bytes per walk launch from FETCH_SIZE + WRITE_SIZE, each scaled by
+    what the counter reads for a device copy of COPY_PROBE_MIB MiB (which
+    fetches and writes exactly that many bytes)."""
+    import tempfile
+    work = tempfile.mkdtemp(prefix="gs_pmc_", dir="/tmp")
+    known_kib = COPY_PROBE_MIB * 1024.0
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        v = _pmc_pass(counter, args, work)
+        scale = known_kib / v["copy"]
+        res[counter] = {"walk_kib": v["walk"], "copy_kib": v["copy"], "scale": scale,
+                        "walk_bytes": v["walk"] * scale * 1024.0}
+    return res
+
+
+def copy_probe(mib, quiet=False):
     """Device-to-device copy of a known size (hipMemcpy), the calibration
     kernel for the FETCH_SIZE / WRITE_SIZE counters."""
     import ctypes
@@ -255,7 +339,8 @@ def copy_probe(mib):
     assert hip.hipDeviceSynchronize() == 0
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
+    if not quiet:
+        print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
 
 
 def main():
@@ -270,12 +355,18 @@ def main():
     ap.add_argument("--lib", default=None, help="path of the libgalaxy_hip.so to load")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure walk HBM traffic")
+    ap.add_argument("--pmc-timeout", type=int, default=120)
+    ap.add_argument("--probe-walk", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-galaxy batched run")
     ap.add_argument("--no-serve", action="store_true", help="skip the served job stream")
     ap.add_argument("--only", choices=("headline", "batch", "serve"), default=None)
+    ap.add_argument("--tuned", default=None, metavar="WG,BUCKET,RADIX",
+                    help="skip the autotuner and use these batched-run parameters")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=5)
-    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-jobs", type=int, default=120)
     ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--serve-resident", type=int, default=8 << 20,
                     help="resident particle limit of the server")
@@ -284,6 +375,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.probe_walk:
+        probe_walk(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -298,7 +392,26 @@ def main():
     if args.only == "serve":
         print(json.dumps(bench_serve(args, {})))
         return
+    traffic = None
+    if not (args.no_pmc or args.only == "headline"):
+        try:
+            log("walk HBM traffic: two rocprofv3 counter passes")
+            traffic = pmc_traffic(args)
+        except Exception as e:
+            log(f"counter passes failed ({type(e).__name__}: {e}); traffic = null")
     out = bench_headline(args)
+    if traffic:
+        fetch = traffic["FETCH_SIZE"]["walk_bytes"]
+        write = traffic["WRITE_SIZE"]["walk_bytes"]
+        out["roofline"]["traffic"] = fetch + write
+        out["roofline"]["traffic_detail"] = {
+            "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_scale": traffic["FETCH_SIZE"]["scale"],
+            "write_scale": traffic["WRITE_SIZE"]["scale"],
+            "algorithmic_bytes": out["roofline"]["walk_algorithmic_bytes"],
+            "hbm_gbs": (fetch + write) / (out["roofline"]["walk_avg_ms"] * 1e-3) / 1e9,
+            "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per walk dispatch, scaled by a "
+                      f"{COPY_PROBE_MIB} MiB device copy"}
     tuned = {}
     if not (args.no_batch or args.only == "headline"):
         try:
@@ -374,6 +487,7 @@ def bench_headline(args):
             "frac": achieved_tflops / FP32_PEAK_TFLOPS, "traffic": None,
             "interactions_per_step": inter, "flop_per_interaction": WALK_FLOP_PER_INTERACTION,
             "walk_avg_ms": walk_avg_ms,
+            "walk_algorithmic_bytes": (68 + 64 * nu) * n,
         },
         "step_hbm": {
             "bytes_per_particle": 236 + 160 * nu, "nu": nu,

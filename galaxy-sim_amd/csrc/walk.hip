@@ -5,15 +5,19 @@
 // ONE node sequence together: the pre-order of the union of their trees, in
 // the order the reference's per-thread stack pops (children pushed octant 0
 // first, so octant 7 is visited first).  Each node record carries its depth
-// and a skip link (the node that follows its subtree), and each lane keeps a
-// bit per depth saying whether it opened the ancestor at that depth.  At a
-// node, the active lanes evaluate the reference's own per-particle decision
-// (nbody_bh.wgsl:97-141): monopole if s^2 < theta^2 d^2 or s^2 < eps^2/4,
-// else leaf direct sum (skipping j == i), else open.  If any lane opens, the
-// wave descends to the last child with those lanes' bit set; otherwise it
-// follows the skip link.  Every lane therefore visits exactly the nodes its
-// own stack would pop, in the same order: its interaction list and its
-// summation order equal the reference walk's.  The node index is
+// and a skip link (the node that follows its subtree).  A lane is active at a
+// node iff it opened every ancestor of the node; in pre-order this is one
+// integer per lane, the active depth limit L: the lane is active at depth d
+// iff d <= L.  (Descending from a node at depth d sets L = d + 1 for the
+// lanes that open it and L = min(L, d) for the others; the ancestors above a
+// later node reached by a skip link are unchanged, so the test stays exact.)
+// At a node, the active lanes evaluate the reference's own per-particle
+// decision (nbody_bh.wgsl:97-141): monopole if s^2 < theta^2 d^2 or
+// s^2 < eps^2/4, else leaf direct sum (skipping j == i), else open.  If any
+// lane opens, the wave descends to the last child; otherwise it follows the
+// skip link.  Every lane therefore visits exactly the nodes its own stack
+// would pop, in the same order: its interaction list and its summation order
+// equal the reference walk's.  The node index is
 // wave-uniform, so node records and leaf members are scalar (SMEM) broadcast
 // loads, there is no stack in memory, and divergence costs only the lanes
 // masked off.
@@ -60,32 +64,32 @@ __device__ __forceinline__ void interact_fast(float m, float rx, float ry, float
   pot = pot - mi;
 }
 
+// leaf member j on particle i (a lane that opened the leaf), skipping j == i
 template <bool EXACT, bool COUNT>
-__device__ __forceinline__ void leaf_pair(float4 q, uint32_t j, uint32_t i, bool need, float4 p,
-                                          float G, float eps2, float& ax, float& ay, float& az,
+__device__ __forceinline__ void leaf_pair(float4 q, uint32_t j, uint32_t i, float4 p, float G,
+                                          float eps2, float& ax, float& ay, float& az,
                                           float& pot, uint32_t& cnt) {
-  const bool take = need && j != i;
-  const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
-  if (EXACT) {
-    if (take) {
+  const bool take = j != i;
+  if (take) {
+    const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
+    if (EXACT) {
       const float dj2 = ((qx * qx + qy * qy) + qz * qz) + eps2;
       interact_exact(G * q.w, qx, qy, qz, dj2, ax, ay, az, pot);
+    } else {
+      const float dj2 =
+          __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, __builtin_fmaf(qz, qz, eps2)));
+      interact_fast(q.w, qx, qy, qz, dj2, ax, ay, az, pot);
     }
-  } else {
-    // branchless: masked-off lanes cost the same issue slot, so they compute
-    // with zero mass instead of branching (keeps the member loads batched)
-    const float dj2 = __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, __builtin_fmaf(qz, qz, eps2)));
-    interact_fast(take ? q.w : 0.0f, qx, qy, qz, dj2, ax, ay, az, pot);
+    if (COUNT) cnt++;
   }
-  if (COUNT && take) cnt++;
 }
 
-// default-path leaf member with a 0/1 lane weight (no index test)
-__device__ __forceinline__ void pair_weighted(float4 q, float w, float4 p, float eps2, float& ax,
-                                              float& ay, float& az, float& pot) {
+// default-path leaf member, no index test (eps > 0: j == i adds zero force)
+__device__ __forceinline__ void pair_fast(float4 q, float4 p, float eps2, float& ax, float& ay,
+                                          float& az, float& pot) {
   const float qx = q.x - p.x, qy = q.y - p.y, qz = q.z - p.z;
   const float dj2 = __builtin_fmaf(qx, qx, __builtin_fmaf(qy, qy, __builtin_fmaf(qz, qz, eps2)));
-  interact_fast(q.w * w, qx, qy, qz, dj2, ax, ay, az, pot);
+  interact_fast(q.w, qx, qy, qz, dj2, ax, ay, az, pot);
 }
 
 __device__ __forceinline__ uint64_t rfl64(uint64_t v) {
@@ -129,8 +133,8 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
   uint32_t cnt = 0;
 
-  uint32_t open_bits = valid ? 1u : 0u;  // bit d: this lane is active at depth d
-  int32_t nd = __ballot(valid) ? 0 : -1;
+  int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
+  int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;
   while (nd >= 0) {
     const NodeRec rec = nodes[nd];  // one 32-byte scalar load
     const float4 c = rec.cm;
@@ -139,7 +143,12 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     const uint32_t depth = gs_meta_depth(meta);
     const int32_t skip = inf.w;
     // a massless node is skipped by every lane (nbody_bh.wgsl:86)
-    const bool act = ((open_bits >> depth) & 1u) && c.w != 0.0f;
+    // a massless node is skipped by every lane (nbody_bh.wgsl:86): its depth
+    // test is made to fail on the scalar unit ((mass bits << 1) == 0)
+    uint32_t mbits;
+    asm("s_lshl_b32 %0, %1, 1" : "=s"(mbits) : "s"(__float_as_uint(c.w)));
+    const int32_t dtest = mbits != 0u ? (int32_t)depth : 64;
+    const bool act = dtest <= alim;
     const float size = __int_as_float(inf.x);
     const float s2 = size * size;
     const bool leaf = gs_meta_is_leaf(meta);
@@ -147,7 +156,14 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     const uint32_t count = leaf ? gs_meta_count(meta) : gs_meta_nc(meta);
     const float rx = c.x - p.x, ry = c.y - p.y, rz = c.z - p.z;
     const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;  // exact: drives the decision
-    const bool mono = act && (s2 < theta2 * d2 || s2 < qeps2);
+    // s2 < theta2*d2 || s2 < eps2/4  <=>  s2 < max(theta2*d2, eps2/4) (maxNum drops a
+    // NaN product, as the || does); inactive lanes compare against NaN, so both
+    // the monopole test and its complement are false for them
+    const float lim = act ? fmaxf(theta2 * d2, qeps2) : __builtin_nanf("");
+    const bool mono = s2 < lim;
+    const bool need = s2 >= lim;  // == act && !mono: s2 and an active lim are never NaN
+    // the same compare as a wave lane mask (ordered >=: the v_cmp result itself)
+    const uint64_t need_mask = __builtin_amdgcn_fcmpf(s2, lim, 3 /* OGE */);
     if (mono) {
       if (EXACT)
         interact_exact(G * c.w, rx, ry, rz, d2, ax, ay, az, pot);
@@ -155,51 +171,45 @@ __global__ __launch_bounds__(1024) void walk_kernel(
         interact_fast(c.w, rx, ry, rz, d2, ax, ay, az, pot);
       if (COUNT) cnt++;
     }
-    const bool need = act && !mono;
+    int32_t next = skip;  // leaves and unopened nodes continue after the subtree
     if (leaf) {
       const bool own = act && i - first < count;
       if (own) dens = (float)count / (s2 * size);
-      if (__ballot(need)) {
+      if (need) {  // lanes opening the leaf; the others are masked off
         const uint32_t last = first + count;
         uint32_t j = first;
         if (!EXACT && !COUNT && eps2 > 0.0f) {
           // eps > 0: the j == i term adds an exact zero to the acceleration
           // (r = 0, f finite), so members run without a per-pair index test
           // and the self potential is taken back once per leaf
-          const float wneed = need ? 1.0f : 0.0f;
-          for (; j + 4 <= last; j += 4) {
-            const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
-            pair_weighted(q0, wneed, p, eps2, ax, ay, az, pot);
-            pair_weighted(q1, wneed, p, eps2, ax, ay, az, pot);
-            pair_weighted(q2, wneed, p, eps2, ax, ay, az, pot);
-            pair_weighted(q3, wneed, p, eps2, ax, ay, az, pot);
-          }
-          for (; j < last; j++) pair_weighted(posm[j], wneed, p, eps2, ax, ay, az, pot);
-          if (need && own) pot = pot + p.w * __builtin_amdgcn_rsqf(eps2);
-        } else {
           for (; j + 4 <= last; j += 4) {  // 4 members per batch of scalar loads
             const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
-            leaf_pair<EXACT, COUNT>(q0, j, i, need, p, G, eps2, ax, ay, az, pot, cnt);
-            leaf_pair<EXACT, COUNT>(q1, j + 1, i, need, p, G, eps2, ax, ay, az, pot, cnt);
-            leaf_pair<EXACT, COUNT>(q2, j + 2, i, need, p, G, eps2, ax, ay, az, pot, cnt);
-            leaf_pair<EXACT, COUNT>(q3, j + 3, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+            pair_fast(q0, p, eps2, ax, ay, az, pot);
+            pair_fast(q1, p, eps2, ax, ay, az, pot);
+            pair_fast(q2, p, eps2, ax, ay, az, pot);
+            pair_fast(q3, p, eps2, ax, ay, az, pot);
+          }
+          for (; j < last; j++) pair_fast(posm[j], p, eps2, ax, ay, az, pot);
+          if (own) pot = pot + p.w * __builtin_amdgcn_rsqf(eps2);
+        } else {
+          for (; j + 4 <= last; j += 4) {
+            const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+            leaf_pair<EXACT, COUNT>(q0, j, i, p, G, eps2, ax, ay, az, pot, cnt);
+            leaf_pair<EXACT, COUNT>(q1, j + 1, i, p, G, eps2, ax, ay, az, pot, cnt);
+            leaf_pair<EXACT, COUNT>(q2, j + 2, i, p, G, eps2, ax, ay, az, pot, cnt);
+            leaf_pair<EXACT, COUNT>(q3, j + 3, i, p, G, eps2, ax, ay, az, pot, cnt);
           }
           for (; j < last; j++)
-            leaf_pair<EXACT, COUNT>(posm[j], j, i, need, p, G, eps2, ax, ay, az, pot, cnt);
+            leaf_pair<EXACT, COUNT>(posm[j], j, i, p, G, eps2, ax, ay, az, pot, cnt);
         }
       }
-      nd = skip;
-    } else {
-      // descend to the last child (visited first) with the opening lanes
-      // active at the children's depth, else continue after the subtree
-      if (__ballot(need)) {
-        const uint32_t bit = 2u << depth;
-        open_bits = need ? (open_bits | bit) : (open_bits & ~bit);
-        nd = (int32_t)(first + count - 1);  // internal: firstChild + nChild - 1
-      } else {
-        nd = skip;
-      }
+    } else if (need_mask) {
+      // descend to the last child (visited first): the opening lanes are
+      // active at the children's depth, the others only above this node
+      alim = need ? (int32_t)depth + 1 : min(alim, (int32_t)depth);
+      next = (int32_t)(first + count - 1);  // internal: firstChild + nChild - 1
     }
+    nd = next;
   }
   if (!EXACT) {
     ax *= G;

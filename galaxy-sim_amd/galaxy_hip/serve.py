@@ -102,6 +102,7 @@ class GalaxyServer:
         self.resident_bytes = 0
         self.step_lat = []
         self.step_particles = []
+        self.step_jobs = []
         self.particle_steps = 0
 
     def close(self):
@@ -162,6 +163,7 @@ class GalaxyServer:
         self.pool.sync()
         self.step_lat.append(time.perf_counter() - t0)
         self.step_particles.append(self.resident_n)
+        self.step_jobs.append(len(self.active))
         self.particle_steps += self.resident_n
         done = []
         for slot, job in list(self.active.items()):
@@ -214,7 +216,11 @@ class GalaxyServer:
         return self.summary(t_end - (first_busy or t_end))
 
     def summary(self, busy_s):
+        """Step latency is reported per job step (each server step counted once
+        for every job it advanced: the latency a job sees per step) and per
+        server step."""
         lat = np.array(self.step_lat) * 1e3
+        jlat = np.repeat(lat, np.array(self.step_jobs, np.int64)) if lat.size else lat
         jl = np.array([j.finished - j.arrival for j in self.finished]) if self.finished else []
         wait = np.array([j.admitted - j.arrival for j in self.finished]) if self.finished else []
         return {
@@ -223,8 +229,10 @@ class GalaxyServer:
             "particle_steps": int(self.particle_steps),
             "busy_s": busy_s,
             "throughput": self.particle_steps / busy_s if busy_s > 0 else 0.0,
-            "step_p50_ms": float(np.percentile(lat, 50)) if lat.size else 0.0,
-            "step_p99_ms": float(np.percentile(lat, 99)) if lat.size else 0.0,
+            "step_p50_ms": float(np.percentile(jlat, 50)) if jlat.size else 0.0,
+            "step_p99_ms": float(np.percentile(jlat, 99)) if jlat.size else 0.0,
+            "server_step_p50_ms": float(np.percentile(lat, 50)) if lat.size else 0.0,
+            "server_step_p99_ms": float(np.percentile(lat, 99)) if lat.size else 0.0,
             "mean_resident_particles": float(np.mean(self.step_particles))
             if self.step_particles else 0.0,
             "job_latency_p50_s": float(np.percentile(jl, 50)) if len(jl) else 0.0,
