@@ -95,6 +95,8 @@ def lib():
         "gs_force": (ctypes.c_int, [_vp]),
         "gs_step": (ctypes.c_int, [_vp, _u32, _u32]),
         "gs_sync": (ctypes.c_int, [_vp]),
+        "gs_apply_permutation": (ctypes.c_int, [_vp, _vp, _u64]),
+        "gs_upload_tree": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u32]),
         "gs_download_particles": (ctypes.c_int, [_vp, _vp, _u64]),
         "gs_read_positions": (ctypes.c_int, [_vp, _vp, _u64]),
         "gs_download_codes": (ctypes.c_int, [_vp, _vp, _u64, ctypes.c_int]),
@@ -224,6 +226,20 @@ class NBodyGPU:
                         sfDensity, sfEff, sfDt, haloCenters)
         self._call("gs_set_params", ctypes.byref(p))
         self.params = p
+
+    def apply_permutation(self, perm):
+        """applyPermutation (NBodyGPU.js:225-238): gather the particle records
+        through perm[new] = old into the other ping-pong buffer."""
+        perm = np.ascontiguousarray(perm, dtype=np.uint32)
+        self._call("gs_apply_permutation", _ptr(perm), perm.size)
+
+    def upload_tree(self, words, num_nodes, level_ranges):
+        """uploadBHTree (NBodyGPU.js:288-301): a host-built tree in the 64-byte
+        node layout for the current particle order."""
+        words = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1)
+        lr = np.ascontiguousarray(np.asarray(level_ranges, np.uint32).reshape(-1))
+        self._call("gs_upload_tree", _ptr(words), int(num_nodes), _ptr(lr) if lr.size else None,
+                   lr.size // 2)
 
     def update_central_mass(self, mass, idx=0):
         self._call("gs_update_central_mass", float(mass), int(idx))

@@ -158,3 +158,38 @@ def test_tiny_pool_makes_the_root_a_leaf():
         sim.destroy()
     ref = oracle_tree(data, bucket=16, max_nodes=5)
     assert info["num_nodes"] == ref["num_nodes"] == 1
+
+
+def test_host_tree_upload_flow_matches_device_sort():
+    """The reference's flow (host sortAndBuild -> applyPermutation ->
+    uploadBHTree -> refresh -> walk) gives the on-device sort's forces."""
+    banner("uploadBHTree flow")
+    data = ic.make("cpu_disk_16k", n=20000)
+    ref = oracle_tree(data, bucket=16)
+    params = dict(dt=0.005, G=1.0, epsilon=0.02, theta=0.6)
+    a = galaxy_hip.NBodyGPU(data, bucket=16, exact_force=True)
+    b = galaxy_hip.NBodyGPU(data, bucket=16, exact_force=True)
+    try:
+        a.update_params(**params)
+        b.update_params(**params)
+        a.apply_permutation(ref["perm"])
+        a.upload_tree(ref["nodes"], ref["num_nodes"], ref["level_ranges"])
+        a.refresh()
+        a.force()
+        b.sort()
+        b.refresh()
+        b.force()
+        assert np.array_equal(a.particles(), b.particles())
+        assert np.array_equal(a.nodes(), b.nodes())
+        acc_a, dens_a = a.accel()
+        acc_b, dens_b = b.accel()
+        assert np.array_equal(acc_a.view(np.uint32), acc_b.view(np.uint32))
+        assert np.array_equal(dens_a.view(np.uint32), dens_b.view(np.uint32))
+        # a child index that does not follow its parent is refused on the host
+        bad = ref["nodes"].copy()
+        bad[8:16] = 0
+        with pytest.raises(galaxy_hip.GalaxyError):
+            a.upload_tree(bad, ref["num_nodes"], ref["level_ranges"])
+    finally:
+        a.destroy()
+        b.destroy()
