@@ -127,6 +127,9 @@ def bench_batch(args):
     pooled = total_n * args.batch_steps / t_pool
     log(f"pooled: {t_pool / args.batch_steps * 1e3:.3f} ms/step, {pooled / 1e6:.1f} M particle-steps/s")
 
+    if args.no_plain:
+        return {"n_total": total_n, "value": pooled, "unit": "particle-steps/s",
+                "ms_per_step": t_pool / args.batch_steps * 1e3}
     # plain form 1: one galaxy after another, each on its own handle
     ksteps = args.plain_steps
     t_seq = 0.0
@@ -245,6 +248,7 @@ def bench_serve(args, tuned):
         "value": summ["throughput"], "unit": "particle-steps/s",
         "resident_limit": args.serve_resident,
         "plain_one_at_a_time": plain, "speedup_vs_plain": summ["throughput"] / plain,
+        "saturated_speedup_vs_plain": summ["saturated_throughput"] / plain,
         "p99_over_p50": summ["step_p99_ms"] / max(summ["step_p50_ms"], 1e-9),
     })
     return summ
@@ -365,6 +369,8 @@ def main():
     ap.add_argument("--tuned", default=None, metavar="WG,BUCKET,RADIX",
                     help="skip the autotuner and use these batched-run parameters")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched run: time the pooled steps only (profiling)")
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--serve-jobs", type=int, default=120)
     ap.add_argument("--serve-steps", type=int, default=200)

@@ -111,7 +111,8 @@ void check_status(gs_sim* s) {
   if (st & GS_ST_NODE_POOL)
     throw Fail{GS_E_NODE_POOL, "octree needs more nodes than the pool holds (raise max_nodes)"};
   if (st & GS_ST_LAYOUT)
-    throw Fail{GS_E_INVALID, "uploaded tree: a node's children are not contiguous"};
+    throw Fail{GS_E_INVALID,
+               "uploaded tree: children not contiguous, or leaf ranges overlap"};
   if (st & GS_ST_STACK) throw Fail{GS_E_STACK, "walk stack overflow"};
   if (st & GS_ST_SPIN) throw Fail{GS_E_DEVICE, "a look-back wait timed out (scan state corrupt)"};
 }
@@ -148,6 +149,7 @@ void free_all(gs_sim* s) {
   f(s->chunk_tab);
   f(s->tile_tab);
   f(s->wblock_slot);
+  f(s->big_leaf);
   for (auto& t : s->timers) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -384,6 +386,9 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     dalloc(&s->chunk_tab, s->chunks, "chunk table");
     dalloc(&s->tile_tab, s->sort_tiles, "tile table");
     dalloc(&s->wblock_slot, s->walk_blocks, "walk block table");
+    // a leaf of more than GS_BIG_LEAF members holds that many particles
+    s->big_cap = (uint32_t)(P / GS_BIG_LEAF + 64);
+    dalloc(&s->big_leaf, s->big_cap, "large-leaf list");
     // zero state with fill kernels (graph-safe, no memset nodes)
     check(gs_launch_fill_u32((uint32_t*)s->sc, 0, sizeof(DevScalars) / 4 * nslots, s->stream),
           "fill");

@@ -112,7 +112,12 @@ struct DevBatch {
   uint32_t status;       // GS_ST_* bits
   uint32_t tickets[32];  // [0..3] sort passes, [4..14] build levels
   uint64_t interactions;
+  uint32_t big_count;    // large leaves queued for the wave-wide refresh pass
+  uint32_t pad_;
 };
+
+// leaves larger than this are refreshed by a whole wavefront each
+#define GS_BIG_LEAF 64
 
 struct __align__(32) NodeRec {
   float4 cm;
@@ -217,6 +222,8 @@ struct gs_sim {
   uint2* chunk_tab = nullptr;       // chunks: (slot, local start)
   uint32_t* tile_tab = nullptr;     // sort_tiles: slot << 16 | local tile
   uint32_t* wblock_slot = nullptr;  // walk_blocks: slot of each walk block
+  uint2* big_leaf = nullptr;        // (slot, node) of large leaves
+  uint32_t big_cap = 0;             // ptot / GS_BIG_LEAF + 64 entries
 
   std::vector<HostSlot> hs;
   std::vector<DevParams> host_prm;

@@ -8,8 +8,9 @@
 // so the stored f32 words equal the reference build's bit-for-bit.
 //
 // Launch plan: one pass over every leaf of every level (a leaf only reads
-// particles), then one launch per internal level, deepest first, each thread
-// summing one node's children (contiguous, octant order).  Kernel boundaries
+// particles; leaves larger than a wavefront are queued and then summed by a
+// wavefront each), then one launch per internal level, deepest first, each
+// thread summing one node's children (contiguous, octant order).  Kernel boundaries
 // order the levels; no device-wide fences or atomics are needed, which on a
 // multi-XCD GPU (8 non-coherent L2s) is far cheaper than per-node
 // release/acquire.  Node counts per level come from device scalars, so the
@@ -21,6 +22,8 @@
 
 namespace {
 
+constexpr int LEAF_BATCH = 8;
+
 __device__ __forceinline__ void store_com(NodeRec* nodes, uint32_t idx, double m, double cx,
                                           double cy, double cz) {
   const double inv = m > 0.0 ? __ddiv_rn(1.0, m) : 0.0;
@@ -28,12 +31,40 @@ __device__ __forceinline__ void store_com(NodeRec* nodes, uint32_t idx, double m
                               (float)__dmul_rn(cz, inv), (float)m);
 }
 
-// every leaf of every slot: FP64 sum over its members in index order
+// FP64 sums over members [j, last) in index order, continuing (m, cx, cy, cz)
+__device__ __forceinline__ void sum_members(const float4* __restrict__ pp, uint32_t j,
+                                            uint32_t last, double& m, double& cx, double& cy,
+                                            double& cz) {
+  // members are loaded LEAF_BATCH at a time ahead of the (sequential,
+  // order-fixed) FP64 sums, so the loads of a batch are in flight together
+  for (uint32_t j0 = j; j0 < last; j0 += LEAF_BATCH) {
+    float4 q[LEAF_BATCH];
+#pragma unroll
+    for (int k = 0; k < LEAF_BATCH; k++)
+      q[k] = j0 + k < last ? pp[j0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < LEAF_BATCH; k++) {
+      if (j0 + k >= last) break;
+      const double pm = (double)q[k].w;
+      m = __dadd_rn(m, pm);
+      cx = __dadd_rn(cx, __dmul_rn((double)q[k].x, pm));
+      cy = __dadd_rn(cy, __dmul_rn((double)q[k].y, pm));
+      cz = __dadd_rn(cz, __dmul_rn((double)q[k].z, pm));
+    }
+  }
+}
+
+// every leaf of every slot: FP64 sum over its members in index order, one
+// thread per node.  Leaves of more than GS_BIG_LEAF members (dense cores at
+// the depth limit hold hundreds, and they sit next to each other in node
+// order) are queued for refresh_big_kernel instead.
 __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __restrict__ posm,
                                                              NodeRec* __restrict__ pool,
                                                              const DevSlot* __restrict__ slots,
                                                              const DevScalars* __restrict__ scs,
-                                                             uint32_t nslots) {
+                                                             uint32_t nslots, DevBatch* batch,
+                                                             uint2* __restrict__ big,
+                                                             uint32_t big_cap) {
   __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
   gs_slot_prefix(threadIdx.x < nslots ? scs[threadIdx.x].num_nodes : 0u, nslots, s_pre);
   const uint32_t total = s_pre[nslots];
@@ -45,18 +76,64 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
     const uint32_t idx = f - s_pre[slot];
     const int4 inf = nodes[idx].info;
     if (!gs_meta_is_leaf((uint32_t)inf.z)) continue;
+    const uint32_t first = (uint32_t)inf.y, count = gs_meta_count((uint32_t)inf.z);
+    if (count > GS_BIG_LEAF) {
+      // disjoint leaves of > GS_BIG_LEAF particles fit the list; overlapping
+      // ranges (only an uploaded tree could have them) are flagged
+      const uint32_t k = atomicAdd(&batch->big_count, 1u);
+      if (k < big_cap) big[k] = make_uint2(slot, idx);
+      else atomicOr(&batch->status, GS_ST_LAYOUT);
+      continue;
+    }
     double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    sum_members(posm + sl.poff, first, first + count, m, cx, cy, cz);
+    store_com(nodes, idx, m, cx, cy, cz);
+  }
+}
+
+// the queued large leaves, one wavefront each: 64 coalesced member loads per
+// chunk staged in LDS, then one lane adds them in index order (the same
+// sequence of roundings as the thread-per-leaf sum)
+__global__ __launch_bounds__(256) void refresh_big_kernel(const float4* __restrict__ posm,
+                                                          NodeRec* __restrict__ pool,
+                                                          const DevSlot* __restrict__ slots,
+                                                          const DevBatch* __restrict__ batch,
+                                                          const uint2* __restrict__ big,
+                                                          uint32_t big_cap) {
+  __shared__ float4 s_mem[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t nbig = min(batch->big_count, big_cap);
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t e = blockIdx.x * (blockDim.x >> 6) + (uint32_t)w; e < nbig; e += waves) {
+    const uint2 b = big[e];
+    const DevSlot sl = slots[b.x];
+    NodeRec* nodes = pool + sl.noff;
+    const int4 inf = nodes[b.y].info;
     const uint32_t first = (uint32_t)inf.y, last = first + gs_meta_count((uint32_t)inf.z);
     const float4* pp = posm + sl.poff;
-    for (uint32_t j = first; j < last; j++) {
-      const float4 p = pp[j];
-      const double pm = (double)p.w;
-      m = __dadd_rn(m, pm);
-      cx = __dadd_rn(cx, __dmul_rn((double)p.x, pm));
-      cy = __dadd_rn(cy, __dmul_rn((double)p.y, pm));
-      cz = __dadd_rn(cz, __dmul_rn((double)p.z, pm));
+    double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (uint32_t c = first; c < last; c += 64) {
+      const uint32_t j = c + (uint32_t)lane;
+      s_mem[w][lane] = j < last ? pp[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane == 0) {
+        const uint32_t nk = last - c < 64u ? last - c : 64u;
+        for (uint32_t t = 0; t < nk; t++) {
+          const float4 q = s_mem[w][t];
+          const double pm = (double)q.w;
+          m = __dadd_rn(m, pm);
+          cx = __dadd_rn(cx, __dmul_rn((double)q.x, pm));
+          cy = __dadd_rn(cy, __dmul_rn((double)q.y, pm));
+          cz = __dadd_rn(cz, __dmul_rn((double)q.z, pm));
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    store_com(nodes, idx, m, cx, cy, cz);
+    if (lane == 0) store_com(nodes, b.y, m, cx, cy, cz);
   }
 }
 
@@ -65,8 +142,12 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
 __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict__ pool,
                                                             const DevSlot* __restrict__ slots,
                                                             const DevScalars* __restrict__ scs,
-                                                            uint32_t nslots, int d) {
+                                                            uint32_t nslots, int d,
+                                                            DevBatch* batch) {
   __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
+  // the large-leaf queue was consumed by the previous launch: empty it for
+  // the next refresh
+  if (d == GS_MAX_LEVEL - 1 && blockIdx.x == 0 && threadIdx.x == 0) batch->big_count = 0;
   gs_slot_prefix(threadIdx.x < nslots ? scs[threadIdx.x].level_count[d] : 0u, nslots, s_pre);
   const uint32_t total = s_pre[nslots];
   for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < total;
@@ -97,8 +178,13 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
   if (g > 8192) g = 8192;
   if (g < 64) g = 64;
   refresh_leaves_kernel<<<(uint32_t)g, 256, 0, s->stream>>>(posm, s->nodes, s->slots, s->sc,
-                                                              s->nslots);
+                                                              s->nslots, s->batch, s->big_leaf,
+                                                              s->big_cap);
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  refresh_big_kernel<<<256, 256, 0, s->stream>>>(posm, s->nodes, s->slots, s->batch, s->big_leaf,
+                                                 s->big_cap);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   // internal nodes exist only at depths 0..9; a level's node count is bounded
   // by nslots * 8^d and by the pools
@@ -107,7 +193,8 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
     if (bound > s->ntot) bound = s->ntot;
     uint32_t gl = (uint32_t)((bound + 255) / 256);
     if (gl > 2048) gl = 2048;
-    refresh_level_kernel<<<gl, 256, 0, s->stream>>>(s->nodes, s->slots, s->sc, s->nslots, d);
+    refresh_level_kernel<<<gl, 256, 0, s->stream>>>(s->nodes, s->slots, s->sc, s->nslots, d,
+                                                     s->batch);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
