@@ -128,6 +128,7 @@ napi_value Create(napi_env env, napi_callback_info info) {
               (num_prop(env, a.v[0], "timing", 0) != 0 ? GS_FLAG_TIMING : 0u);
   cfg.device = (int32_t)num_prop(env, a.v[0], "device", -1);
   cfg.walk_wg = (uint32_t)num_prop(env, a.v[0], "walkWg", 0);
+  cfg.radix_bits = (uint32_t)num_prop(env, a.v[0], "radixBits", 0);
   gs_sim* s = nullptr;
   int rc = gs_create(&cfg, &s);
   if (rc != GS_OK) return throw_gs(env, nullptr, rc, "gs_create");
@@ -164,13 +165,9 @@ napi_value Upload(napi_env env, napi_callback_info info) {
   return rc ? throw_gs(env, s, rc, "gs_upload") : nullptr;
 }
 
-napi_value SetParams(napi_env env, napi_callback_info info) {
-  Args a;
-  get_args(env, info, a);
-  SIM_OR_RETURN(s, a.v[0]);
-  gs_params p;
+// updateParams dictionary (NBodyGPU.js:253-275) -> gs_params
+void parse_params(napi_env env, napi_value o, gs_params& p) {
   gs_params_default(&p);
-  napi_value o = a.v[1];
   p.dt = num_prop(env, o, "dt", p.dt);
   p.G = num_prop(env, o, "G", p.G);
   p.epsilon = num_prop(env, o, "epsilon", p.epsilon);
@@ -206,6 +203,14 @@ napi_value SetParams(napi_env env, napi_callback_info info) {
       }
     }
   }
+}
+
+napi_value SetParams(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  gs_params p;
+  parse_params(env, a.v[1], p);
   int rc = gs_set_params(s, &p);
   return rc ? throw_gs(env, s, rc, "gs_set_params") : nullptr;
 }
@@ -457,6 +462,202 @@ napi_value TreeBuild(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// ---- slot pools (gs_create_pool / gs_slot_*) --------------------------------
+uint32_t u32_arg(napi_env env, napi_value v) {
+  double d = 0;
+  napi_get_value_double(env, v, &d);
+  return (uint32_t)d;
+}
+
+// capacities / buckets: Uint32Array or Array of numbers
+bool u32_list(napi_env env, napi_value v, std::string& err, uint32_t* out, uint32_t cap,
+              uint32_t* n) {
+  bool isarr = false;
+  napi_is_array(env, v, &isarr);
+  if (isarr) {
+    uint32_t len = 0;
+    napi_get_array_length(env, v, &len);
+    if (len > cap) {
+      err = "too many slots";
+      return false;
+    }
+    for (uint32_t k = 0; k < len; k++) {
+      napi_value e;
+      napi_get_element(env, v, k, &e);
+      out[k] = u32_arg(env, e);
+    }
+    *n = len;
+    return true;
+  }
+  napi_typedarray_type t;
+  size_t length = 0;
+  void* data = nullptr;
+  napi_value ab;
+  size_t off = 0;
+  if (napi_get_typedarray_info(env, v, &t, &length, &data, &ab, &off) != napi_ok ||
+      t != napi_uint32_array || length > cap) {
+    err = "expected an Array or Uint32Array of at most 256 values";
+    return false;
+  }
+  memcpy(out, data, 4 * length);
+  *n = (uint32_t)length;
+  return true;
+}
+
+// createPool({capacities, buckets, exactForce, device, walkWg, radixBits, graph}) -> handle
+napi_value CreatePool(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 1) {
+    napi_throw_type_error(env, nullptr, "createPool(options)");
+    return nullptr;
+  }
+  napi_value o = a.v[0];
+  uint32_t caps[256] = {0}, buckets[256] = {0}, nc = 0, nb = 0;
+  std::string err;
+  bool has = false;
+  napi_has_named_property(env, o, "capacities", &has);
+  if (!has) {
+    napi_throw_type_error(env, nullptr, "createPool: capacities required");
+    return nullptr;
+  }
+  napi_value v;
+  napi_get_named_property(env, o, "capacities", &v);
+  if (!u32_list(env, v, err, caps, 256, &nc)) {
+    napi_throw_type_error(env, nullptr, err.c_str());
+    return nullptr;
+  }
+  napi_has_named_property(env, o, "buckets", &has);
+  if (has) {
+    napi_get_named_property(env, o, "buckets", &v);
+    if (!u32_list(env, v, err, buckets, 256, &nb) || nb != nc) {
+      napi_throw_type_error(env, nullptr, "createPool: buckets must match capacities");
+      return nullptr;
+    }
+  }
+  gs_pool_config cfg;
+  memset(&cfg, 0, sizeof cfg);
+  cfg.nslots = nc;
+  cfg.capacity = caps;
+  cfg.bucket = nb ? buckets : nullptr;
+  cfg.flags = (num_prop(env, o, "exactForce", 0) != 0 ? GS_FLAG_EXACT_FORCE : 0u) |
+              (num_prop(env, o, "graph", 1) == 0 ? GS_FLAG_NO_GRAPH : 0u);
+  cfg.device = (int32_t)num_prop(env, o, "device", -1);
+  cfg.walk_wg = (uint32_t)num_prop(env, o, "walkWg", 0);
+  cfg.radix_bits = (uint32_t)num_prop(env, o, "radixBits", 0);
+  gs_sim* s = nullptr;
+  int rc = gs_create_pool(&cfg, &s);
+  if (rc != GS_OK) return throw_gs(env, nullptr, rc, "gs_create_pool");
+  Holder* h = new Holder{s};
+  napi_value ext;
+  CALL(env, napi_create_external(env, h, finalize_holder, nullptr, &ext));
+  return ext;
+}
+
+// slotUpload(h, slot, Float32Array records)
+napi_value SlotUpload(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  size_t len = 0;
+  float* p = typed<float>(env, a.v[2], napi_float32_array, &len);
+  if (!p) return nullptr;
+  int rc = gs_slot_upload(s, u32_arg(env, a.v[1]), p, len / 8);
+  return rc ? throw_gs(env, s, rc, "gs_slot_upload") : nullptr;
+}
+
+napi_value SlotClear(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  int rc = gs_slot_clear(s, u32_arg(env, a.v[1]));
+  return rc ? throw_gs(env, s, rc, "gs_slot_clear") : nullptr;
+}
+
+napi_value SlotCount(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  uint32_t n = 0;
+  int rc = gs_slot_count(s, u32_arg(env, a.v[1]), &n);
+  return rc ? throw_gs(env, s, rc, "gs_slot_count") : num(env, n);
+}
+
+// slotSetParams(h, slot, updateParams dictionary)
+napi_value SlotSetParams(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  gs_params p;
+  parse_params(env, a.v[2], p);
+  int rc = gs_slot_set_params(s, u32_arg(env, a.v[1]), &p);
+  return rc ? throw_gs(env, s, rc, "gs_slot_set_params") : nullptr;
+}
+
+// slotSetTree(h, slot, bucket, maxNodes)
+napi_value SlotSetTree(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  double mn = 0;
+  if (a.n > 3) napi_get_value_double(env, a.v[3], &mn);
+  int rc = gs_slot_set_tree(s, u32_arg(env, a.v[1]), u32_arg(env, a.v[2]), (uint64_t)mn);
+  return rc ? throw_gs(env, s, rc, "gs_slot_set_tree") : nullptr;
+}
+
+// slotDownloadParticles(h, slot, Float32Array(8 n))
+napi_value SlotDownloadParticles(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  size_t len = 0;
+  float* p = typed<float>(env, a.v[2], napi_float32_array, &len);
+  if (!p) return nullptr;
+  int rc = gs_slot_download_particles(s, u32_arg(env, a.v[1]), p, len / 8);
+  return rc ? throw_gs(env, s, rc, "gs_slot_download_particles") : nullptr;
+}
+
+// slotDownloadEnergy(h, slot, Float32Array(n))
+napi_value SlotDownloadEnergy(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  size_t len = 0;
+  float* p = typed<float>(env, a.v[2], napi_float32_array, &len);
+  if (!p) return nullptr;
+  int rc = gs_slot_download_energy(s, u32_arg(env, a.v[1]), p, len);
+  return rc ? throw_gs(env, s, rc, "gs_slot_download_energy") : nullptr;
+}
+
+// slotReduce(h, slot) -> Float64Array(GS_NRED)
+napi_value SlotReduce(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  void* data = nullptr;
+  napi_value ab, arr;
+  CALL(env, napi_create_arraybuffer(env, GS_NRED * 8, &data, &ab));
+  int rc = gs_slot_reduce(s, u32_arg(env, a.v[1]), static_cast<double*>(data));
+  if (rc) return throw_gs(env, s, rc, "gs_slot_reduce");
+  CALL(env, napi_create_typedarray(env, napi_float64_array, GS_NRED, ab, 0, &arr));
+  return arr;
+}
+
+// poolInfo(h) -> {nslots, deviceBytes}
+napi_value PoolInfo(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  uint32_t ns = 0;
+  uint64_t bytes = 0;
+  int rc = gs_pool_info(s, &ns, &bytes);
+  if (rc) return throw_gs(env, s, rc, "gs_pool_info");
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "nslots", num(env, ns));
+  napi_set_named_property(env, o, "deviceBytes", num(env, (double)bytes));
+  return o;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value v;
   napi_create_string_utf8(env, gs_version(), NAPI_AUTO_LENGTH, &v);
@@ -539,6 +740,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"treeInfo", TreeInfo}, {"reduce", Reduce}, {"countInteractions", CountInteractions},
       {"computeMortonCodes", ComputeMortonCodes}, {"radixSortIndices", RadixSortIndices},
       {"treeBuild", TreeBuild}, {"version", Version}, {"deviceCount", DeviceCount},
+      {"createPool", CreatePool}, {"poolInfo", PoolInfo}, {"slotUpload", SlotUpload},
+      {"slotClear", SlotClear}, {"slotCount", SlotCount}, {"slotSetParams", SlotSetParams},
+      {"slotSetTree", SlotSetTree}, {"slotDownloadParticles", SlotDownloadParticles},
+      {"slotDownloadEnergy", SlotDownloadEnergy}, {"slotReduce", SlotReduce},
   };
   for (auto& f : fns) {
     napi_value fn;

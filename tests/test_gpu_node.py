@@ -19,3 +19,13 @@ def test_node_shim_parity():
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stderr[-2000:]
     assert "node shim parity ok" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed on this machine")
+def test_node_pool_parity():
+    """GalaxyPool.js slots equal NBodyGPU.js run alone, bit for bit."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "node", "pool_parity.mjs")],
+                       capture_output=True, text=True, timeout=240, cwd=ROOT)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "node pool parity ok" in r.stdout
