@@ -117,10 +117,18 @@ void check_status(gs_sim* s) {
   if (st & GS_ST_SPIN) throw Fail{GS_E_DEVICE, "a look-back wait timed out (scan state corrupt)"};
 }
 
-void free_all(gs_sim* s) {
+void destroy_graphs(gs_sim* s) {
   for (int p = 0; p < 2; p++)
     for (int q = 0; q < 2; q++)
-      if (s->graph[p][q]) (void)hipGraphExecDestroy(s->graph[p][q]);
+      for (int r = 0; r < 2; r++)
+        if (s->graph[p][q][r]) {
+          (void)hipGraphExecDestroy(s->graph[p][q][r]);
+          s->graph[p][q][r] = nullptr;
+        }
+}
+
+void free_all(gs_sim* s) {
+  destroy_graphs(s);
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -130,6 +138,7 @@ void free_all(gs_sim* s) {
     f(s->keys[k]);
     f(s->vals[k]);
     f(s->build_status[k]);
+    f(s->stel[k]);
   }
   f(s->codes_raw);
   f(s->sort_status);
@@ -211,6 +220,7 @@ void launch_sort(gs_sim* s, bool keep_raw) {
     check(gs_launch_sort(s, true), "radix sort");
   }
   s->cur ^= 1;
+  if (s->stel[0]) s->stel_cur ^= 1;  // the gather moved the stellar records too
   {
     Stage t(s, GS_T_BUILD);
     check(gs_launch_build(s), "build");
@@ -250,9 +260,9 @@ bool use_graph(const gs_sim* s) { return !(s->flags & (GS_FLAG_NO_GRAPH | GS_FLA
 // ping-pong parity; the launch sequence depends only on static sizes, so one
 // capture serves every later step and every change of slot occupancy.
 hipGraphExec_t step_graph(gs_sim* s, bool sort) {
-  hipGraphExec_t& slot = s->graph[s->cur][sort ? 1 : 0];
+  hipGraphExec_t& slot = s->graph[s->cur][s->stel_cur][sort ? 1 : 0];
   if (slot) return slot;
-  const int cur0 = s->cur;
+  const int cur0 = s->cur, stel0 = s->stel_cur;
   hipGraph_t g = nullptr;
   check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   try {
@@ -264,10 +274,12 @@ hipGraphExec_t step_graph(gs_sim* s, bool sort) {
     (void)hipStreamEndCapture(s->stream, &dead);
     if (dead) (void)hipGraphDestroy(dead);
     s->cur = cur0;
+    s->stel_cur = stel0;
     throw;
   }
   check(hipStreamEndCapture(s->stream, &g), "end capture");
   s->cur = cur0;
+  s->stel_cur = stel0;
   hipGraphExec_t ex = nullptr;
   hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -281,6 +293,7 @@ void run_step(gs_sim* s, bool sort) {
     hipGraphExec_t ex = step_graph(s, sort);
     check(hipGraphLaunch(ex, s->stream), "graph launch");
     s->cur ^= sort ? 0 : 1;  // the sort's gather flips once, the walk once
+    if (sort && s->stel[0]) s->stel_cur ^= 1;
   } else {
     Stage t(s, GS_T_STEP);
     if (sort) launch_sort(s, false);
@@ -448,6 +461,10 @@ void upload_particles(gs_sim* s, uint32_t slot, const float* particles, uint64_t
   need(particles != nullptr || n == 0, GS_E_INVALID, "null particles");
   need(n <= h.d.cap, GS_E_INVALID, "more particles than the slot holds");
   check(gs_upload_records(s, slot, particles, (uint32_t)n), "upload particles");
+  if (s->stel[0])  // a new galaxy starts without stellar records until it uploads its own
+    for (int k = 0; k < 2; k++)
+      check(gs_launch_fill_u32((uint32_t*)(s->stel[k] + h.d.poff), 0, 4ull * h.d.cap, s->stream),
+            "clear stellar");
   h.d.n = (uint32_t)n;
   if (!h.fixed_max_nodes)  // new MortonTree(N): maxNodes = 2N + 4096 (mortonTree.js:88)
     h.d.max_nodes = std::min<uint64_t>(2 * n + 4096, h.d.ncap);
@@ -455,6 +472,18 @@ void upload_particles(gs_sim* s, uint32_t slot, const float* particles, uint64_t
   push_params(s, slot);
   sync(s);
   h.have_tree = h.have_force = h.have_step = h.have_codes = false;
+}
+
+// Stellar records are allocated on first upload; step graphs captured
+// without them are dropped so the next capture gathers them too.
+void ensure_stellar(gs_sim* s) {
+  if (s->stel[0]) return;
+  for (int k = 0; k < 2; k++) {
+    dalloc(&s->stel[k], s->ptot, "stellar records");
+    check(gs_launch_fill_u32((uint32_t*)s->stel[k], 0, 4 * s->ptot, s->stream), "fill");
+  }
+  s->stel_cur = 0;
+  destroy_graphs(s);
 }
 
 }  // namespace
@@ -659,6 +688,7 @@ int gs_apply_permutation(gs_sim* s, const uint32_t* perm, uint64_t n) {
     (void)hipFree(dperm);
     check(e, "gather");
     s->cur ^= 1;
+    if (s->stel[0]) s->stel_cur ^= 1;
     for (auto& hh : s->hs) hh.have_force = false;
   });
 }
@@ -898,6 +928,44 @@ int gs_slot_reduce(gs_sim* s, uint32_t slot, double out[GS_NRED]) {
                          s->stream),
           "reduce");
     sync(s);
+  });
+}
+
+int gs_upload_stellar(gs_sim* s, const float* stellar4, uint64_t n) {
+  return gs_slot_upload_stellar(s, 0, stellar4, n);
+}
+
+int gs_slot_upload_stellar(gs_sim* s, uint32_t slot, const float* stellar4, uint64_t n) {
+  return guarded(s, [&] {
+    const HostSlot& h = slot_of(s, slot);
+    need(stellar4 != nullptr && n == h.d.n, GS_E_INVALID,
+         "stellar records must match the slot's particles (n x 4 f32)");
+    ensure_stellar(s);
+    if (n)
+      check(hipMemcpyAsync(s->stel[s->stel_cur] + h.d.poff, stellar4, 16 * n,
+                           hipMemcpyHostToDevice, s->stream),
+            "stellar upload");
+    sync(s);
+  });
+}
+
+int gs_download_stellar(gs_sim* s, float* dst4, uint64_t n) {
+  return gs_slot_download_stellar(s, 0, dst4, n);
+}
+
+int gs_slot_download_stellar(gs_sim* s, uint32_t slot, float* dst4, uint64_t n) {
+  return guarded(s, [&] {
+    const HostSlot& h = slot_of(s, slot);
+    need(dst4 != nullptr && n == h.d.n, GS_E_INVALID, "stellar download needs n x 4 f32");
+    need(s->stel[0] != nullptr, GS_E_STATE, "no stellar records uploaded");
+    download_slot(s, slot, dst4, s->stel[s->stel_cur], 16, n, "stellar download");
+  });
+}
+
+int gs_star_formation(gs_sim* s) {
+  return guarded(s, [&] {
+    need(s->stel[0] != nullptr, GS_E_STATE, "star formation needs stellar records");
+    check(gs_launch_star_formation(s), "star formation");
   });
 }
 

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
   __shared__ uint32_t s_b[GS_BUILD_TILE][9];
   __shared__ uint32_t s_wsum[GS_BUILD_TILE / 64];
   __shared__ uint64_t s_prefix;
-  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
+  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1 + GS_MAX_SLOTS / 64];
 
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t k = gtid; k < clear_n; k += gridDim.x * blockDim.x) status_clear[k] = 0;
@@ -316,7 +316,8 @@ __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nod
 }
 
 // gather through a host permutation for one slot; every other slot is copied
-// unchanged so the whole pool moves to the other ping-pong buffer
+// unchanged so the whole pool moves to the other ping-pong buffer (stellar
+// records too, when present)
 __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict__ perm,
                                                      uint32_t target,
                                                      const DevSlot* __restrict__ slots,
@@ -325,7 +326,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict_
                                                      const float4* __restrict__ pin,
                                                      const float4* __restrict__ qin,
                                                      float4* __restrict__ pout,
-                                                     float4* __restrict__ qout) {
+                                                     float4* __restrict__ qout,
+                                                     const float4* __restrict__ sin,
+                                                     float4* __restrict__ sout) {
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ptot;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t slot = wblock_slot[p / wg];
@@ -335,6 +338,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict_
     if (slot == target && local < sl.n) src = perm[local];
     pout[p] = pin[sl.poff + src];
     qout[p] = qin[sl.poff + src];
+    if (sin) sout[p] = sin[sl.poff + src];
   }
 }
 }  // namespace
@@ -363,6 +367,7 @@ hipError_t gs_launch_gather(gs_sim* s, uint32_t slot, const uint32_t* perm) {
   gather_kernel<<<(uint32_t)g, 256, 0, s->stream>>>(perm, slot, s->slots, s->wblock_slot,
                                                      s->walk_wg, s->ptot, s->posm[s->cur],
                                                      s->velp[s->cur], s->posm[s->cur ^ 1],
-                                                     s->velp[s->cur ^ 1]);
+                                                     s->velp[s->cur ^ 1], s->stel[s->stel_cur],
+                                                     s->stel[s->stel_cur ^ 1]);
   return hipGetLastError();
 }

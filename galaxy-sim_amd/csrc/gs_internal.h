@@ -144,15 +144,30 @@ __host__ __device__ __forceinline__ uint32_t gs_meta_count(uint32_t m) { return 
 __host__ __device__ __forceinline__ uint32_t gs_meta_nc(uint32_t m) { return m & 0xfu; }
 __host__ __device__ __forceinline__ uint32_t gs_meta_mask(uint32_t m) { return (m >> 8) & 0xffu; }
 
-// Block-wide exclusive prefix over per-slot values (one thread per slot,
-// nslots <= blockDim.x <= 1024); result in s_pre[0..nslots], s_pre[nslots] =
-// total.  All threads must call it.
+// Block-wide exclusive prefix over per-slot values (thread t holds slot t's
+// value; nslots <= GS_MAX_SLOTS <= blockDim.x, blockDim.x a multiple of 64);
+// result in s_pre[0..nslots], s_pre[nslots] = total.  Wavefront scans of the
+// first GS_MAX_SLOTS / 64 waves, then the wave totals.  All threads must call
+// it; s_pre needs GS_MAX_SLOTS + 1 + GS_MAX_SLOTS / 64 words.
 __device__ __forceinline__ void gs_slot_prefix(uint32_t v, uint32_t nslots, uint32_t* s_pre) {
-  if (threadIdx.x < nslots) s_pre[threadIdx.x + 1] = v;
-  if (threadIdx.x == 0) s_pre[0] = 0;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  uint32_t* s_wave = s_pre + GS_MAX_SLOTS + 1;
+  uint32_t x = t < nslots ? v : 0u;
+  if (t < GS_MAX_SLOTS) {
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
+  }
   __syncthreads();
-  if (threadIdx.x == 0)  // serial: nslots <= 256
-    for (uint32_t k = 1; k <= nslots; k++) s_pre[k] += s_pre[k - 1];
+  if (t < GS_MAX_SLOTS) {
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < w; k++) off += s_wave[k];
+    if (t < nslots) s_pre[t + 1] = off + x;
+    if (t == 0) s_pre[0] = 0;
+  }
   __syncthreads();
 }
 
@@ -201,6 +216,10 @@ struct gs_sim {
   float4* posm[2] = {nullptr, nullptr};
   float4* velp[2] = {nullptr, nullptr};
   int cur = 0;
+  // stellar records (mass, birth Myr, seed, dust), allocated on first upload;
+  // their ping-pong flips only when the particles are permuted
+  float4* stel[2] = {nullptr, nullptr};
+  int stel_cur = 0;
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* codes_raw = nullptr;
@@ -233,7 +252,7 @@ struct gs_sim {
   double t_ms[GS_NT] = {0};
   uint64_t t_cnt[GS_NT] = {0};
 
-  hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][sort]
+  hipGraphExec_t graph[2][2][2] = {};  // [particle parity][stellar parity][sort]
   bool graph_dirty = true;
 
   std::string err;
@@ -260,6 +279,7 @@ hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, c
                               uint32_t num);
 hipError_t gs_launch_gather(gs_sim* s, uint32_t slot, const uint32_t* perm);
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
+hipError_t gs_launch_star_formation(gs_sim* s);  // every slot, in place
 // 32-byte host records <-> (posm, velp) of one slot, staged contiguously in
 // the slot's range of the non-current ping-pong buffers (stream-ordered)
 hipError_t gs_upload_records(gs_sim* s, uint32_t slot, const float* host, uint32_t n);

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
                                                              uint32_t nslots, DevBatch* batch,
                                                              uint2* __restrict__ big,
                                                              uint32_t big_cap) {
-  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
+  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1 + GS_MAX_SLOTS / 64];
   gs_slot_prefix(threadIdx.x < nslots ? scs[threadIdx.x].num_nodes : 0u, nslots, s_pre);
   const uint32_t total = s_pre[nslots];
   for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < total;
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
                                                             const DevScalars* __restrict__ scs,
                                                             uint32_t nslots, int d,
                                                             DevBatch* batch) {
-  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1];
+  __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1 + GS_MAX_SLOTS / 64];
   // the large-leaf queue was consumed by the previous launch: empty it for
   // the next refresh
   if (d == GS_MAX_LEVEL - 1 && blockIdx.x == 0 && threadIdx.x == 0) batch->big_count = 0;

@@ -66,7 +66,8 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int pass, uint32_t* status,
     DevBatch* batch, const DevSlot* __restrict__ slots, const DevScalars* __restrict__ scs,
     const uint32_t* __restrict__ tile_tab, const float4* __restrict__ pin,
-    const float4* __restrict__ qin, float4* __restrict__ pout, float4* __restrict__ qout) {
+    const float4* __restrict__ qin, float4* __restrict__ pout, float4* __restrict__ qout,
+    const float4* __restrict__ sin, float4* __restrict__ sout) {
   constexpr int ITEMS = GS_SORT_ITEMS;
   constexpr uint32_t RADIX = 1u << BITS;
   constexpr int DPT = (int)RADIX / GS_SORT_BLOCK;  // digits per thread
@@ -201,6 +202,7 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
       if (GATHER && v < n) {
         pout[sl.poff + out] = pin[sl.poff + v];
         qout[sl.poff + out] = qin[sl.poff + v];
+        if (sin) sout[sl.poff + out] = sin[sl.poff + v];  // stellar records, when present
       }
     }
   }
@@ -216,11 +218,11 @@ hipError_t launch_sort(gs_sim* s, bool gather) {
       onesweep_kernel<BITS, true><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
           s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
           s->tile_tab, s->posm[s->cur], s->velp[s->cur], s->posm[s->cur ^ 1],
-          s->velp[s->cur ^ 1]);
+          s->velp[s->cur ^ 1], s->stel[s->stel_cur], s->stel[s->stel_cur ^ 1]);
     else
       onesweep_kernel<BITS, false><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
           s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
-          s->tile_tab, nullptr, nullptr, nullptr, nullptr);
+          s->tile_tab, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

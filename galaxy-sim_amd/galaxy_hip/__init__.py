@@ -124,6 +124,11 @@ def lib():
         "gs_slot_download_nodes": (ctypes.c_int, [_vp, _u32, _vp, _u64, ctypes.POINTER(_u64)]),
         "gs_slot_download_energy": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
         "gs_slot_reduce": (ctypes.c_int, [_vp, _u32, _vp]),
+        "gs_upload_stellar": (ctypes.c_int, [_vp, _vp, _u64]),
+        "gs_download_stellar": (ctypes.c_int, [_vp, _vp, _u64]),
+        "gs_slot_upload_stellar": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
+        "gs_slot_download_stellar": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
+        "gs_star_formation": (ctypes.c_int, [_vp]),
         "gs_compute_morton_codes": (ctypes.c_int, [_vp, _vp, _u32, _u32, _vp,
                                                    ctypes.POINTER(ctypes.c_double)]),
         "gs_radix_sort_indices": (ctypes.c_int, [_vp, _vp, _u32, _vp]),
@@ -240,6 +245,22 @@ class NBodyGPU:
         lr = np.ascontiguousarray(np.asarray(level_ranges, np.uint32).reshape(-1))
         self._call("gs_upload_tree", _ptr(words), int(num_nodes), _ptr(lr) if lr.size else None,
                    lr.size // 2)
+
+    def upload_stellar(self, stellar):
+        """The stellar records (NBodyGPU.js:83-90): N x 4 f32 [mass, birth Myr,
+        seed, dust] in the current particle order."""
+        stellar = np.ascontiguousarray(stellar, dtype=np.float32).reshape(-1, 4)
+        self._call("gs_upload_stellar", _ptr(stellar), stellar.shape[0])
+
+    def stellar(self):
+        """stellarBuffer (NBodyGPU.js:241-243), in the current particle order."""
+        out = np.empty((self.N, 4), np.float32)
+        self._call("gs_download_stellar", _ptr(out), self.N)
+        return out
+
+    def star_formation(self):
+        """encodeStarFormation (NBodyGPU.js:277-285)."""
+        self._call("gs_star_formation")
 
     def update_central_mass(self, mass, idx=0):
         self._call("gs_update_central_mass", float(mass), int(idx))
@@ -386,6 +407,20 @@ class GalaxyPool:
     def clear(self, slot):
         self._call("gs_slot_clear", slot)
         self.counts[slot] = 0
+
+    def upload_stellar(self, slot, stellar):
+        stellar = np.ascontiguousarray(stellar, dtype=np.float32).reshape(-1, 4)
+        self._call("gs_slot_upload_stellar", slot, _ptr(stellar), stellar.shape[0])
+
+    def stellar(self, slot):
+        n = int(self.counts[slot])
+        out = np.empty((n, 4), np.float32)
+        if n:
+            self._call("gs_slot_download_stellar", slot, _ptr(out), n)
+        return out
+
+    def star_formation(self):
+        self._call("gs_star_formation")
 
     def count(self, slot):
         n = _u32(0)

@@ -8,8 +8,8 @@
 //     `commandEncoder` argument and ignores it; work is queued on the
 //     simulation's own HIP stream in call order, which is the order a WebGPU
 //     queue would execute the encoded passes.
-//   - stellar state (the 16-byte render record) is kept host-side and gathered
-//     on applyPermutation only; star formation is not part of this library.
+//   - stellarBuffer returns a host copy of the stellar records; they live on
+//     the device and follow every permutation (sort or applyPermutation).
 //   - sort() and stepN() expose the on-device rebuild (codes, sort, build).
 import { addon } from './binding.js';
 
@@ -41,7 +41,9 @@ export class NBodyGPU {
       device: this.device && typeof this.device.ordinal === 'number' ? this.device.ordinal : -1,
     });
     addon.upload(this.h, particleData);
-    this.stellar = stellarData ? new Float32Array(stellarData) : null;
+    // stellar records live on the device next to the particles (NBodyGPU.js:83-90)
+    this.hasStellar = !!stellarData;
+    if (stellarData) addon.uploadStellar(this.h, new Float32Array(stellarData));
     this.levelRanges = null;
     this.ready = true;
   }
@@ -49,20 +51,18 @@ export class NBodyGPU {
   /** applyPermutation (NBodyGPU.js:225-238): gather by a host permutation. */
   applyPermutation(commandEncoder, perm) {
     if (!this.ready) return;
-    addon.applyPermutation(this.h, perm);
-    if (this.stellar) {
-      const src = this.stellar, dst = new Float32Array(src.length);
-      for (let i = 0; i < perm.length; i++) {
-        const o = i * 4, q = perm[i] * 4;
-        dst[o] = src[q]; dst[o + 1] = src[q + 1]; dst[o + 2] = src[q + 2]; dst[o + 3] = src[q + 3];
-      }
-      this.stellar = dst;
-    }
+    addon.applyPermutation(this.h, perm);  // particles and stellar records
     this.currentBuffer ^= 1;
     this.stellarCurrent ^= 1;
   }
 
-  get stellarBuffer() { return this.stellar; }
+  /** stellarBuffer (NBodyGPU.js:241-243): N x 4 f32 in the current order (host copy). */
+  get stellarBuffer() {
+    if (!this.hasStellar) return null;
+    const out = new Float32Array(this.N * 4);
+    addon.downloadStellar(this.h, out);
+    return out;
+  }
 
   updateCentralMass(mass, idx) {
     addon.updateCentralMass(this.h, mass, idx === undefined ? 0 : idx);
@@ -85,9 +85,10 @@ export class NBodyGPU {
     });
   }
 
-  /** Star formation is outside this library (nbody_bh.wgsl:229-268). */
+  /** encodeStarFormation (NBodyGPU.js:277-285): star_formation in place. */
   encodeStarFormation(commandEncoder) {
-    throw new Error('galaxy_hip: star_formation is not implemented (gravity path only)');
+    if (!this.ready || !this.hasStellar) return;
+    addon.starFormation(this.h);
   }
 
   /** uploadBHTree (NBodyGPU.js:288-301): tree built by mortonTree.build. */

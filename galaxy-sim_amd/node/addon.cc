@@ -658,6 +658,36 @@ napi_value PoolInfo(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// uploadStellar(h, Float32Array(4 n)) / slotUploadStellar(h, slot, Float32Array)
+napi_value UploadStellar(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  const bool slotted = a.n > 2;
+  size_t len = 0;
+  float* p = typed<float>(env, a.v[slotted ? 2 : 1], napi_float32_array, &len);
+  if (!p) return nullptr;
+  int rc = slotted ? gs_slot_upload_stellar(s, u32_arg(env, a.v[1]), p, len / 4)
+                   : gs_upload_stellar(s, p, len / 4);
+  return rc ? throw_gs(env, s, rc, "gs_upload_stellar") : nullptr;
+}
+
+// downloadStellar(h, Float32Array(4 n)) / slotDownloadStellar(h, slot, Float32Array)
+napi_value DownloadStellar(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  const bool slotted = a.n > 2;
+  size_t len = 0;
+  float* p = typed<float>(env, a.v[slotted ? 2 : 1], napi_float32_array, &len);
+  if (!p) return nullptr;
+  int rc = slotted ? gs_slot_download_stellar(s, u32_arg(env, a.v[1]), p, len / 4)
+                   : gs_download_stellar(s, p, len / 4);
+  return rc ? throw_gs(env, s, rc, "gs_download_stellar") : nullptr;
+}
+
+SIMPLE(StarFormation, gs_star_formation)
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value v;
   napi_create_string_utf8(env, gs_version(), NAPI_AUTO_LENGTH, &v);
@@ -744,6 +774,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"slotClear", SlotClear}, {"slotCount", SlotCount}, {"slotSetParams", SlotSetParams},
       {"slotSetTree", SlotSetTree}, {"slotDownloadParticles", SlotDownloadParticles},
       {"slotDownloadEnergy", SlotDownloadEnergy}, {"slotReduce", SlotReduce},
+      {"uploadStellar", UploadStellar}, {"slotUploadStellar", UploadStellar},
+      {"downloadStellar", DownloadStellar}, {"slotDownloadStellar", DownloadStellar},
+      {"starFormation", StarFormation},
   };
   for (auto& f : fns) {
     napi_value fn;

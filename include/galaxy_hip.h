@@ -176,6 +176,18 @@ int gs_download_energy(gs_sim* sim, float* dst, uint64_t n);
 /* Result of the last gs_force: n x (ax, ay, az, pot), and leaf density
  * (either pointer may be NULL). */
 int gs_download_accel(gs_sim* sim, float* acc4, float* density, uint64_t n);
+/* Stellar records (NBodyGPU.js:83-90, stellarBuffer :241-243): n x 4 f32
+ * [stellar mass, birth time (Myr), seed, dust] in the current particle order.
+ * Allocated on first upload; the sort's gather and gs_apply_permutation carry
+ * them along (apply_permutation_stellar, nbody_bh.wgsl:336-340); a new
+ * particle upload into a slot clears that slot's records. */
+int gs_upload_stellar(gs_sim* sim, const float* stellar4, uint64_t n);
+int gs_download_stellar(gs_sim* sim, float* dst4, uint64_t n);
+/* encodeStarFormation (NBodyGPU.js:277-285, star_formation
+ * nbody_bh.wgsl:229-268): gas denser than sfDensity turns into stars, massive
+ * stars past their supernova return to gas with a kick; every occupied slot,
+ * in place, with each slot's time / agingRate / sfDensity / sfEff / sfDt. */
+int gs_star_formation(gs_sim* sim);
 /* FP64 reductions of the last step (GS_RED_*). */
 int gs_reduce(gs_sim* sim, double out[GS_NRED]);
 /* Summed stage times (GS_T_*) since the last reset; needs GS_FLAG_TIMING.
@@ -235,6 +247,10 @@ int gs_slot_download_nodes(gs_sim* sim, uint32_t slot, uint32_t* dst,
 int gs_slot_download_energy(gs_sim* sim, uint32_t slot, float* dst,
                             uint64_t n);
 int gs_slot_reduce(gs_sim* sim, uint32_t slot, double out[GS_NRED]);
+int gs_slot_upload_stellar(gs_sim* sim, uint32_t slot, const float* stellar4,
+                           uint64_t n);
+int gs_slot_download_stellar(gs_sim* sim, uint32_t slot, float* dst4,
+                             uint64_t n);
 
 /* ---- mortonTree.js on host arrays (device-executed, copies in and out) --- */
 /* computeMortonCodes(pos, N, stride, codes) -> rootSize (mortonTree.js:43). */

@@ -100,6 +100,12 @@ void or_direct_sum_f64(const float* pin, uint32_t n, double G, double eps2,
                        const uint32_t* idx, uint32_t m, double* acc3,
                        int nthreads);
 
+/* star_formation (nbody_bh.wgsl:197-268) */
+uint32_t or_pcg(uint32_t v0);
+float or_rand01(uint32_t i, uint32_t salt);
+float or_imf_sample(float u1, float u2);
+void or_star_formation(float* particles, float* stellar, uint32_t n, const or_params* p);
+
 #ifdef __cplusplus
 }
 #endif

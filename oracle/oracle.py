@@ -58,6 +58,12 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB)
         L.or_part1by2.restype = ctypes.c_uint32
+        L.or_pcg.restype = ctypes.c_uint32
+        L.or_pcg.argtypes = [ctypes.c_uint32]
+        L.or_rand01.restype = ctypes.c_float
+        L.or_rand01.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.or_imf_sample.restype = ctypes.c_float
+        L.or_imf_sample.argtypes = [ctypes.c_float, ctypes.c_float]
         L.or_part1by2.argtypes = [ctypes.c_uint32]
         L.or_js_toint32.restype = ctypes.c_int32
         L.or_js_toint32.argtypes = [ctypes.c_double]
@@ -184,13 +190,41 @@ def sort_and_build(particles, n, stellar=None, track_idx2=-1, bucket=16, max_nod
 
 
 def pack_params(n, dt=0.005, G=1.0, epsilon=0.02, theta=0.7, halo_mass=0.0, halo_a=1.0,
-                gas_drag=0.0, halo_centers=()):
+                gas_drag=0.0, halo_centers=(), time=0.0, aging_rate=25.0, sf_density=1e9,
+                sf_eff=0.0, sf_dt=0.0):
     p = Params()
     c1 = (ctypes.c_double * 3)(*(halo_centers[0][:3] if len(halo_centers) > 0 else (0, 0, 0)))
     c2 = (ctypes.c_double * 3)(*(halo_centers[1][:3] if len(halo_centers) > 1 else (0, 0, 0)))
     lib().or_pack_params(ctypes.byref(p), dt, G, epsilon, theta, halo_mass, halo_a, gas_drag,
                          n, min(len(halo_centers), 2), c1, c2)
+    # the remaining updateParams fields, each a Float32Array store (NBodyGPU.js:253-275)
+    p.time, p.aging_rate, p.sf_density = float(time), float(aging_rate), float(sf_density)
+    p.sf_eff, p.sf_dt = float(sf_eff), float(sf_dt)
     return p
+
+
+def pcg(v):
+    """pcg (nbody_bh.wgsl:206-210) over an array of u32."""
+    v = np.ascontiguousarray(v, dtype=np.uint32)
+    return np.array([lib().or_pcg(int(x)) for x in v], np.uint32)
+
+
+def rand01(i, salt):
+    i = np.ascontiguousarray(i, dtype=np.uint32)
+    return np.array([lib().or_rand01(int(x), int(salt)) for x in i], np.float32)
+
+
+def imf_sample(u1, u2):
+    return np.float32(lib().or_imf_sample(float(u1), float(u2)))
+
+
+def star_formation(particles, stellar, params):
+    """star_formation (nbody_bh.wgsl:229-268) on copies: (particles, stellar)."""
+    pr = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1).copy()
+    st = np.ascontiguousarray(stellar, dtype=np.float32).reshape(-1).copy()
+    n = pr.size // 8
+    lib().or_star_formation(_p(pr, f32p), _p(st, f32p), n, ctypes.byref(params))
+    return pr.reshape(n, 8), st.reshape(n, 4)
 
 
 def walk(particles, nodes, params, nthreads=None):

@@ -382,3 +382,73 @@ void or_direct_sum_f64(const float* pin, uint32_t n, double G, double eps2,
   free(th);
   free(jobs);
 }
+
+/* ---- star_formation (nbody_bh.wgsl:197-268) ------------------------------
+ * pcg (nbody_bh.wgsl:206-210): the PCG-RXS-M-XS 32-bit hash — one step of
+ * the 32-bit PCG LCG (multiplier 747796405, increment 2891336453) followed by
+ * the RXS-M-XS output permutation (multiplier 277803737). */
+uint32_t or_pcg(uint32_t v0) {
+  uint32_t v = v0 * 747796405u + 2891336453u;
+  v = ((v >> ((v >> 28u) + 4u)) ^ v) * 277803737u;
+  return (v >> 22u) ^ v;
+}
+
+/* rand01 (nbody_bh.wgsl:211-213): f32(pcg(i ^ salt * 0x9E3779B9)) / f32(4294967295.0) */
+float or_rand01(uint32_t i, uint32_t salt) {
+  return (float)or_pcg(i ^ (salt * 0x9E3779B9u)) / (float)4294967295.0;
+}
+
+/* imfSample (nbody_bh.wgsl:216-227).  The shader's pow(0.08, -0.3) etc. are
+ * constant expressions: evaluated here in double and rounded to f32 once; the
+ * rest is f32 arithmetic, left to right, with the library powf. */
+float or_imf_sample(float u1, float u2) {
+  const float a0 = (float)pow(0.08, -0.3);
+  const float b0 = (float)pow(0.5, -0.3);
+  const float lowInt = (a0 - b0) / 0.3f;
+  const float a1 = (float)pow(0.5, -1.3);
+  const float b1 = (float)pow(100.0, -1.3);
+  const float hiInt = 0.5f * (a1 - b1) / 1.3f;
+  if (u1 < lowInt / (lowInt + hiInt)) return powf(a0 - u2 * (a0 - b0), (float)(-1.0 / 0.3));
+  return powf(a1 - u2 * (a1 - b1), (float)(-1.0 / 1.3));
+}
+
+/* star_formation (nbody_bh.wgsl:229-268), in place on n particle records
+ * (8 f32) and stellar records (4 f32). */
+void or_star_formation(float* particles, float* stellar, uint32_t n, const or_params* p) {
+  const uint32_t salt = (uint32_t)(p->time * 977.0f) + 1u;
+  for (uint32_t i = 0; i < n; i++) {
+    float* q = particles + (size_t)i * 8;
+    float* st = stellar + (size_t)i * 4;
+    if (q[7] < -0.5f) {
+      const float density = -q[7] - 1.0f;
+      if (density > p->sf_density && p->sf_eff > 0.0f) {
+        const float prob = p->sf_eff * p->sf_dt *
+                           fminf(sqrtf(density / fmaxf(p->sf_density, 1e-6f)), 3.0f);
+        if (or_rand01(i, salt) < prob) {
+          const float m = or_imf_sample(or_rand01(i, salt + 7u), or_rand01(i, salt + 13u));
+          const float birthMyr = p->time * p->aging_rate;
+          st[0] = m;
+          st[1] = birthMyr;
+          st[2] = or_rand01(i, salt + 29u);
+          st[3] = 0.0f;
+          q[7] = 0.0f;
+        }
+      }
+    } else if (st[0] >= 8.0f) {
+      const float ageMyr = p->time * p->aging_rate - st[1];
+      const float tMS = 10000.0f * powf(st[0], -2.5f);
+      const float tGiant = tMS * 0.12f + 15.0f;
+      if (ageMyr > tMS + tGiant + 15.0f + 30.0f) {
+        const float seed = st[2];
+        st[0] = -1.0f;
+        st[1] = 0.0f;
+        st[2] = seed;
+        st[3] = 0.7f;
+        q[7] = -1.0f;
+        q[4] += (or_rand01(i, salt + 41u) - 0.5f) * 2.0f;
+        q[5] += (or_rand01(i, salt + 43u) - 0.5f) * 2.0f;
+        q[6] += (or_rand01(i, salt + 47u) - 0.5f) * 2.0f;
+      }
+    }
+  }
+}

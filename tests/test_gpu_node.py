@@ -29,3 +29,13 @@ def test_node_pool_parity():
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stderr[-2000:]
     assert "node pool parity ok" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed on this machine")
+def test_node_stellar_flow():
+    """stellarBuffer follows applyPermutation; encodeStarFormation forms stars."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "node", "stellar_flow.mjs")],
+                       capture_output=True, text=True, timeout=240, cwd=ROOT)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "node stellar flow ok" in r.stdout
