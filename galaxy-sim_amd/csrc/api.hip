@@ -962,6 +962,31 @@ int gs_slot_download_stellar(gs_sim* s, uint32_t slot, float* dst4, uint64_t n) 
   });
 }
 
+int gs_locate(gs_sim* s, uint32_t old_index, uint32_t* new_index, float pos4[4]) {
+  return gs_slot_locate(s, 0, old_index, new_index, pos4);
+}
+
+int gs_slot_locate(gs_sim* s, uint32_t slot, uint32_t old_index, uint32_t* new_index,
+                   float pos4[4]) {
+  return guarded(s, [&] {
+    const HostSlot& h = slot_of(s, slot);
+    need(h.have_tree, GS_E_STATE, "no sort yet");
+    need(old_index < h.d.n, GS_E_INVALID, "index out of range");
+    check(gs_launch_locate(s, slot, old_index, &s->batch->located), "locate");
+    uint32_t ni = 0xffffffffu;
+    check(hipMemcpyAsync(&ni, &s->batch->located, 4, hipMemcpyDeviceToHost, s->stream), "locate");
+    sync(s);
+    need(ni < h.d.n, GS_E_STATE, "index not found in the last permutation");
+    if (new_index) *new_index = ni;
+    if (pos4) {
+      check(hipMemcpyAsync(pos4, s->posm[s->cur] + h.d.poff + ni, 16, hipMemcpyDeviceToHost,
+                           s->stream),
+            "locate position");
+      sync(s);
+    }
+  });
+}
+
 int gs_star_formation(gs_sim* s) {
   return guarded(s, [&] {
     need(s->stel[0] != nullptr, GS_E_STATE, "star formation needs stellar records");

@@ -113,7 +113,7 @@ struct DevBatch {
   uint32_t tickets[32];  // [0..3] sort passes, [4..14] build levels
   uint64_t interactions;
   uint32_t big_count;    // large leaves queued for the wave-wide refresh pass
-  uint32_t pad_;
+  uint32_t located;      // gs_locate's answer
 };
 
 // leaves larger than this are refreshed by a whole wavefront each
@@ -285,3 +285,4 @@ hipError_t gs_launch_star_formation(gs_sim* s);  // every slot, in place
 hipError_t gs_upload_records(gs_sim* s, uint32_t slot, const float* host, uint32_t n);
 hipError_t gs_download_records(gs_sim* s, uint32_t slot, float* host, uint32_t n);
 hipError_t gs_download_positions(gs_sim* s, uint32_t slot, float* host4, uint32_t n);
+hipError_t gs_launch_locate(gs_sim* s, uint32_t slot, uint32_t old, uint32_t* out_dev);

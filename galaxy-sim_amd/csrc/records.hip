@@ -55,6 +55,15 @@ __global__ __launch_bounds__(256) void pack_positions_kernel(const float4* __res
   }
 }
 
+// where the particle that had index `old` before the last sort is now:
+// perm[new] = old (remapIndex of the frame loop, main.js:436-440)
+__global__ __launch_bounds__(256) void locate_kernel(const uint32_t* __restrict__ perm,
+                                                     uint32_t n, uint32_t old,
+                                                     uint32_t* __restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (perm[i] == old) *out = i;
+}
+
 inline uint32_t grid_of(uint32_t n) {
   uint32_t g = (n + 255) / 256;
   if (g > 8192) g = 8192;
@@ -104,4 +113,13 @@ hipError_t gs_download_positions(gs_sim* s, uint32_t slot, float* host4, uint32_
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return hipMemcpyAsync(host4, sa, 16ull * n, hipMemcpyDeviceToHost, s->stream);
+}
+
+hipError_t gs_launch_locate(gs_sim* s, uint32_t slot, uint32_t old, uint32_t* out_dev) {
+  const DevSlot& d = s->hs[slot].d;
+  hipError_t e = gs_launch_fill_u32(out_dev, 0xffffffffu, 1, s->stream);
+  if (e != hipSuccess) return e;
+  locate_kernel<<<grid_of(d.n), 256, 0, s->stream>>>(s->vals[gs_sorted_buf(s)] + d.poff, d.n, old,
+                                                      out_dev);
+  return hipGetLastError();
 }

@@ -126,6 +126,10 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   const NodeRec* __restrict__ nodes = pool + sl.noff;
   const uint32_t i = (blk - sl.wblock0) * blockDim.x + threadIdx.x;  // slot-local particle
   const bool valid = i < n;
+  if ((blk - sl.wblock0) * blockDim.x >= n) {  // a block of empty slot capacity
+    if (STEP && threadIdx.x < GS_NRED) red[(size_t)blk * GS_NRED + threadIdx.x] = 0.0;
+    return;
+  }
   const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
   const float qeps2 = 0.25f * eps2;

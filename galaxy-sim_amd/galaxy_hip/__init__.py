@@ -129,6 +129,8 @@ def lib():
         "gs_slot_upload_stellar": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
         "gs_slot_download_stellar": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
         "gs_star_formation": (ctypes.c_int, [_vp]),
+        "gs_locate": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(_u32), _vp]),
+        "gs_slot_locate": (ctypes.c_int, [_vp, _u32, _u32, ctypes.POINTER(_u32), _vp]),
         "gs_compute_morton_codes": (ctypes.c_int, [_vp, _vp, _u32, _u32, _vp,
                                                    ctypes.POINTER(ctypes.c_double)]),
         "gs_radix_sort_indices": (ctypes.c_int, [_vp, _vp, _u32, _vp]),
@@ -262,6 +264,14 @@ class NBodyGPU:
         """encodeStarFormation (NBodyGPU.js:277-285)."""
         self._call("gs_star_formation")
 
+    def locate(self, old_index):
+        """remapIndex (main.js:436-440): current index and (x, y, z, m) of the
+        particle that had old_index before the last sort."""
+        ni = _u32(0)
+        pos = np.zeros(4, np.float32)
+        self._call("gs_locate", int(old_index), ctypes.byref(ni), _ptr(pos))
+        return ni.value, pos
+
     def update_central_mass(self, mass, idx=0):
         self._call("gs_update_central_mass", float(mass), int(idx))
 
@@ -352,6 +362,16 @@ class NBodyGPU:
         self._call("gs_count_interactions", ctypes.byref(tot),
                    _ptr(pp) if pp is not None else None)
         return (tot.value, pp) if per_particle else tot.value
+
+
+def track_halo_centers(centers, targets, k=0.1):
+    """Halo-centre EMA of the frame loop (main.js:449-457): each centre moves
+    the fraction k toward its target (the tracked central black hole), in
+    double precision, in place; returns the centres."""
+    for h in range(min(len(centers), len(targets))):
+        for c in range(3):
+            centers[h][c] += (float(targets[h][c]) - centers[h][c]) * k
+    return centers
 
 
 class GalaxyPool:

@@ -19,6 +19,15 @@ export async function requestGPUDevice() {
   return { ordinal: 0, kind: 'hip-gfx950' };
 }
 
+/** Halo-centre EMA of the frame loop (main.js:449-457), in place. */
+export function trackHaloCenters(haloCenters, targets, k) {
+  const kk = k === undefined ? 0.1 : k;
+  for (let h = 0; h < haloCenters.length && h < targets.length; h++) {
+    for (let c = 0; c < 3; c++) haloCenters[h][c] += (targets[h][c] - haloCenters[h][c]) * kk;
+  }
+  return haloCenters;
+}
+
 export class NBodyGPU {
   constructor(device, particleData, stellarData, options) {
     this.device = device;
@@ -140,6 +149,9 @@ export class NBodyGPU {
     addon.downloadParticles(this.h, out);
     return out;
   }
+
+  /** remapIndex (main.js:436-440) after an on-device rebuild: {index, pos}. */
+  locate(oldIndex) { return addon.locate(this.h, oldIndex); }
 
   treeInfo() { return addon.treeInfo(this.h); }
   downloadNodes() { return addon.downloadNodes(this.h); }

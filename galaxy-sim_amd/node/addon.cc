@@ -688,6 +688,24 @@ napi_value DownloadStellar(napi_env env, napi_callback_info info) {
 
 SIMPLE(StarFormation, gs_star_formation)
 
+// locate(h, oldIndex) -> {index, pos: Float32Array(4)}
+napi_value Locate(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  uint32_t ni = 0;
+  void* data = nullptr;
+  napi_value ab, pos, o;
+  CALL(env, napi_create_arraybuffer(env, 16, &data, &ab));
+  int rc = gs_locate(s, u32_arg(env, a.v[1]), &ni, static_cast<float*>(data));
+  if (rc) return throw_gs(env, s, rc, "gs_locate");
+  CALL(env, napi_create_typedarray(env, napi_float32_array, 4, ab, 0, &pos));
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "index", num(env, ni));
+  napi_set_named_property(env, o, "pos", pos);
+  return o;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value v;
   napi_create_string_utf8(env, gs_version(), NAPI_AUTO_LENGTH, &v);
@@ -776,7 +794,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"slotDownloadEnergy", SlotDownloadEnergy}, {"slotReduce", SlotReduce},
       {"uploadStellar", UploadStellar}, {"slotUploadStellar", UploadStellar},
       {"downloadStellar", DownloadStellar}, {"slotDownloadStellar", DownloadStellar},
-      {"starFormation", StarFormation},
+      {"starFormation", StarFormation}, {"locate", Locate},
   };
   for (auto& f : fns) {
     napi_value fn;

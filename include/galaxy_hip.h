@@ -188,6 +188,13 @@ int gs_download_stellar(gs_sim* sim, float* dst4, uint64_t n);
  * stars past their supernova return to gas with a kick; every occupied slot,
  * in place, with each slot's time / agingRate / sfDensity / sfEff / sfDt. */
 int gs_star_formation(gs_sim* sim);
+/* remapIndex (main.js:436-440) on the device: the current index of the
+ * particle that had index old_index before the last gs_sort (perm[new] =
+ * old), and its (x, y, z, m) now (pos4 may be NULL).  The frame loop uses it
+ * to follow the central black holes and drift the halo centres toward them
+ * (main.js:449-457). */
+int gs_locate(gs_sim* sim, uint32_t old_index, uint32_t* new_index,
+              float pos4[4]);
 /* FP64 reductions of the last step (GS_RED_*). */
 int gs_reduce(gs_sim* sim, double out[GS_NRED]);
 /* Summed stage times (GS_T_*) since the last reset; needs GS_FLAG_TIMING.
@@ -247,6 +254,8 @@ int gs_slot_download_nodes(gs_sim* sim, uint32_t slot, uint32_t* dst,
 int gs_slot_download_energy(gs_sim* sim, uint32_t slot, float* dst,
                             uint64_t n);
 int gs_slot_reduce(gs_sim* sim, uint32_t slot, double out[GS_NRED]);
+int gs_slot_locate(gs_sim* sim, uint32_t slot, uint32_t old_index,
+                   uint32_t* new_index, float pos4[4]);
 int gs_slot_upload_stellar(gs_sim* sim, uint32_t slot, const float* stellar4,
                            uint64_t n);
 int gs_slot_download_stellar(gs_sim* sim, uint32_t slot, float* dst4,

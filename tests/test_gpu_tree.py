@@ -193,3 +193,26 @@ def test_host_tree_upload_flow_matches_device_sort():
     finally:
         a.destroy()
         b.destroy()
+
+
+def test_locate_follows_particles_through_the_sort():
+    """gs_locate = remapIndex (main.js:436-440): the new index of an old one."""
+    banner("locate")
+    data = ic.make("collision_4m", n=30000)
+    sim = galaxy_hip.NBodyGPU(data, bucket=32)
+    try:
+        sim.update_params(dt=0.005, G=1.0, epsilon=0.005, theta=0.5)
+        sim.sort()
+        perm = sim.perm()
+        parts = sim.particles()
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(perm.size, dtype=np.uint32)
+        for old in (0, 1, 15000, 29999):
+            new, pos = sim.locate(old)
+            assert new == inv[old]
+            assert np.array_equal(pos, parts[new, :4])
+            assert np.array_equal(pos, data[old, :4])
+        with pytest.raises(galaxy_hip.GalaxyError):
+            sim.locate(30000)
+    finally:
+        sim.destroy()
