@@ -35,20 +35,6 @@ __device__ __forceinline__ void st_agent64(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ uint32_t lower_bound(const uint32_t* __restrict__ codes, uint32_t first,
-                                                uint32_t len, uint32_t target) {
-  while (len > 0) {
-    const uint32_t half = len >> 1;
-    if (codes[first + half] < target) {
-      first += half + 1;
-      len -= half + 1;
-    } else {
-      len = half;
-    }
-  }
-  return first;
-}
-
 __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const uint32_t* __restrict__ keys, NodeRec* __restrict__ pool,
     const DevSlot* __restrict__ slots, DevScalars* scs, DevBatch* batch, uint32_t nslots, int d,
@@ -74,6 +60,10 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
   }
   gs_slot_prefix(my_tiles, nslots, s_pre);
   const uint32_t total_tiles = s_pre[nslots];
+  // the grid is sized for the largest level: blocks beyond this level's tile
+  // count leave without touching the ticket counter (tickets keep the
+  // look-back order; the blocks that stay still take every tile)
+  if (blockIdx.x >= total_tiles) return;
   const int shift = 27 - 3 * d;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 
@@ -98,23 +88,56 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const uint32_t size_d = __float_as_uint(sc->level_size[d]);
     const uint32_t size_c = __float_as_uint(sc->level_size[d + 1]);
 
-    // phase 1: octant boundaries b_0..b_7 (+ hi), 8 lanes per node
-    for (int r = 0; r < GS_BUILD_TILE / 32; r++) {
-      const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
+    // phase 1: octant boundaries b_0..b_7 (+ hi), 8 lanes per node; each
+    // thread owns one octant of R nodes and advances their R binary searches
+    // in lockstep, so the dependent loads of the searches overlap
+    {
+      constexpr int R = GS_BUILD_TILE / 32;
       const uint32_t k = threadIdx.x & 7;
-      const uint32_t local = tile * GS_BUILD_TILE + j;
-      if (local < count) {
-        const uint32_t idx = start + local;
-        const int4 inf = nodes[idx].info;
-        const uint32_t lo = (uint32_t)inf.y, cnt = gs_meta_count((uint32_t)inf.z);
-        const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
-        uint32_t b = lo;
-        if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
-          const uint32_t pre = (codes[lo] >> (shift + 3)) << (shift + 3);
-          b = lower_bound(codes, lo, cnt, pre | (k << shift));
+      uint32_t sfirst[R], slen[R], stgt[R];
+      bool more = false;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
+        const uint32_t local = tile * GS_BUILD_TILE + j;
+        sfirst[r] = 0;
+        slen[r] = 0;
+        stgt[r] = 0;
+        if (local < count) {
+          const uint32_t idx = start + local;
+          const int4 inf = nodes[idx].info;
+          const uint32_t lo = (uint32_t)inf.y, cnt = gs_meta_count((uint32_t)inf.z);
+          const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
+          sfirst[r] = lo;
+          if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
+            slen[r] = cnt;
+            stgt[r] = ((codes[lo] >> (shift + 3)) << (shift + 3)) | (k << shift);
+            more = true;
+          }
+          if (k == 0) s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
         }
-        s_b[j][k] = b;
-        if (k == 0) s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
+      }
+      // lower_bound over [first, first + len) for every r at once
+      while (more) {
+        more = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (slen[r] > 0) {
+            const uint32_t half = slen[r] >> 1;
+            if (codes[sfirst[r] + half] < stgt[r]) {
+              sfirst[r] += half + 1;
+              slen[r] -= half + 1;
+            } else {
+              slen[r] = half;
+            }
+            more |= slen[r] > 0;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
+        if (tile * GS_BUILD_TILE + j < count) s_b[j][k] = sfirst[r];
       }
     }
     __syncthreads();
@@ -154,34 +177,44 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     }
     const uint32_t excl_local = woff + x - nchild;
 
-    if (threadIdx.x == 0) {
+    if (w == 0) {
+      // decoupled look-back, 64 predecessors per load round: lane l reads
+      // tile j - l; the nearest inclusive prefix (kPre) ends the walk once
+      // every tile before it in the window has published its aggregate
       uint64_t* st = st_base + tile;
       uint64_t prefix = 0;
       if (tile == 0) {
-        st_agent64(st, kPre | tile_total);
+        if (lane == 0) st_agent64(st, kPre | tile_total);
       } else {
-        st_agent64(st, kAgg | tile_total);
+        if (lane == 0) st_agent64(st, kAgg | tile_total);
         int32_t j = (int32_t)tile - 1;
         uint32_t spins = 0;
         for (;;) {
-          const uint64_t v = ld_agent64(st_base + j);
+          const int32_t jj = j - lane;
+          // tile 0 always publishes kPre, so lanes below it are never summed
+          const uint64_t v = jj >= 0 ? ld_agent64(st_base + jj) : kPre;
           const uint64_t f = v & 0xffffffff00000000ull;
-          if (f == 0) {
+          const uint64_t pre = __ballot(f == kPre), none = __ballot(f == 0);
+          const uint64_t upto = pre ? (pre & (0 - pre)) * 2 - 1 : ~0ull;  // lanes <= first kPre
+          if (none & upto) {
             if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
-              atomicOr(&batch->status, GS_ST_SPIN);
+              if (lane == 0) atomicOr(&batch->status, GS_ST_SPIN);
               break;
             }
             __builtin_amdgcn_s_sleep(1);
             continue;
           }
-          prefix += v & 0xffffffffull;
-          if (f == kPre) break;
-          j--;
+          uint64_t x = ((upto >> lane) & 1) ? (v & 0xffffffffull) : 0;
+#pragma unroll
+          for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+          prefix += x;
+          if (pre) break;
+          j -= 64;
         }
-        st_agent64(st, kPre | (prefix + tile_total));
+        if (lane == 0) st_agent64(st, kPre | (prefix + tile_total));
       }
-      s_prefix = prefix;
-      if (tile == ntiles - 1) {
+      if (lane == 0) s_prefix = prefix;
+      if (lane == 0 && tile == ntiles - 1) {
         uint64_t total = prefix + tile_total;
         uint64_t cnt_next = total;
         if ((uint64_t)next_start + total > cap) {

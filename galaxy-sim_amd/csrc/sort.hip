@@ -152,22 +152,38 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     const uint32_t d = (uint32_t)tid * DPT + j;
     uint32_t excl = 0;
     if (tile > 0) {
-      int32_t jt = (int32_t)gtile - 1;  // a slot's tiles are consecutive
+      // a slot's tiles are consecutive; LB predecessors are read per round
+      // (independent loads in flight together) and consumed nearest first
+      // up to the first inclusive prefix or the first unpublished tile
+      constexpr int LB = 8;
+      const int32_t first_gt = (int32_t)(gtile - tile);
+      int32_t jt = (int32_t)gtile - 1;
       uint32_t spins = 0;
       for (;;) {
-        const uint32_t v = ld_agent(status + (size_t)jt * RADIX + d);
-        const uint32_t f = v & ~kVal;
-        if (f == 0) {
+        uint32_t v[LB];
+#pragma unroll
+        for (int q = 0; q < LB; q++)
+          v[q] = jt - q >= first_gt ? ld_agent(status + (size_t)(jt - q) * RADIX + d) : kPre;
+        int q = 0;
+        bool done = false;
+        for (; q < LB; q++) {
+          const uint32_t f = v[q] & ~kVal;
+          if (f == 0) break;
+          excl += v[q] & kVal;
+          if (f == kPre) {
+            done = true;
+            break;
+          }
+        }
+        if (done) break;
+        jt -= q;
+        if (q == 0) {  // the nearest unread tile has not published yet
           if (++spins > GS_SPIN_LIMIT) {  // bounded wait: flag and stop, never hang
             atomicOr(&batch->status, GS_ST_SPIN);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
-          continue;
         }
-        excl += v & kVal;
-        if (f == kPre) break;
-        jt--;
       }
       st_agent(st[j], kPre | (excl + total[j]));
     }
