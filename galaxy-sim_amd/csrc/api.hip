@@ -328,7 +328,9 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     d.n = 0;
     d.cap = cap;
     d.poff = (uint32_t)poff;
-    d.ncap = (uint32_t)std::max<uint64_t>(2ull * cap + 4096, max_nodes);  // mortonTree.js:88
+    // mortonTree.js:88, capped at GS_NODE32_MAX records
+    d.ncap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(2ull * cap + 4096, max_nodes),
+                                          GS_NODE32_MAX);
     d.noff = (uint32_t)noff;
     d.bucket = buckets && buckets[k] ? buckets[k] : 16;
     d.max_nodes = d.ncap;
@@ -540,7 +542,7 @@ int gs_create(const gs_config* cfg, gs_sim** out) {
   const uint32_t bucket = cfg->bucket ? cfg->bucket : 16;
   gs_sim* s = nullptr;
   try {
-    if (cfg->max_nodes >= (1ull << 31)) return GS_E_INVALID;
+    if (cfg->max_nodes > GS_NODE32_MAX) return GS_E_INVALID;
     s = make_pool(&cfg->n, &bucket, 1, cfg->flags, cfg->device, wg,
                   cfg->radix_bits ? cfg->radix_bits : 8, cfg->max_nodes);
   } catch (...) {

@@ -15,7 +15,7 @@
 // exclusive scan of child counts over the level; the scan runs in the same
 // kernel with a decoupled look-back over 256-node tiles taken in ticket
 // order.  A node's children are therefore contiguous: internal nodes store
-// (firstChild, nChild, octant mask) and the 64-byte layout is produced on
+// (lastChild, nChild, octant mask) and the 64-byte layout is produced on
 // download.  The last tile of a level writes count_{d+1}.
 //
 // One launch per level serves every slot of the pool: a block's ticket is
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
             c++;
           }
         }
-        nodes[idx].info = make_int4((int)size_d, (int)fc,
+        nodes[idx].info = make_int4((int)size_d, (int)(fc + nchild - 1),  // last child
                                      (int)gs_meta_internal((uint32_t)d, mask, nchild), skip);
       }
     }
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void node64_kernel(const NodeRec* __restrict__
     w[6] = leaf ? (uint32_t)inf.y : 0u;
     w[7] = leaf ? gs_meta_count(meta) : 0u;
     const uint32_t mask = leaf ? 0u : gs_meta_mask(meta);
-    uint32_t c0 = (uint32_t)inf.y;
+    uint32_t c0 = (uint32_t)inf.y + 1u - gs_meta_nc(meta);  // internal: y = last child
     for (int k = 0; k < 8; k++) {
       if (mask & (1u << k)) {
         w[8 + k] = c0++;
@@ -343,8 +343,9 @@ __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nod
         nc++;
       }
     }
-    nodes[i].info = make_int4((int)b.x, (int)(nc ? fc : 0u), (int)gs_meta_internal(depth, mask, nc),
-                              skip);
+    // internal: y = last child (a childless node continues at its skip)
+    nodes[i].info = make_int4((int)b.x, nc ? (int)(fc + nc - 1) : skip,
+                              (int)gs_meta_internal(depth, mask, nc), skip);
   }
 }
 

@@ -17,7 +17,7 @@
 //   keys[2][P], vals[2][P]   u32         radix-sort ping-pong (code, index)
 //   node[C]     32-byte record (one s_load_dwordx8 in the walk):
 //               float4 cm   (comx, comy, comz, mass)
-//               int4   info (size bits, first | firstChild, meta, skip)
+//               int4   info (size bits, first | lastChild, meta, skip)
 //               meta: bit 31 leaf, bits 27..30 depth, then
 //                     leaf: bits 0..26 particle count
 //                     internal: bits 0..3 child count, bits 8..15 octant mask
@@ -25,9 +25,11 @@
 //                     walk's pre-order (previous sibling, else the parent's
 //                     skip; -1 ends the walk)
 // Node and particle indices inside a slot are slot-local (0-based).  A slot's
-// node pool holds >= 2 cap + 4096 nodes (mortonTree.js:88).  Children of a
-// BFS node are contiguous, so an internal node stores (firstChild, nChild,
-// mask) instead of 8 slots; gs_download_nodes expands to the 64-byte layout.
+// node pool holds min(2 cap + 4096, 2^27) nodes (mortonTree.js:88).  Children
+// of a BFS node are contiguous, so an internal node stores (lastChild, nChild,
+// mask) instead of 8 slots (the walk descends to the last child first);
+// gs_download_nodes expands to the 64-byte layout.  A childless internal node
+// (only an uploaded tree can hold one) stores its skip as lastChild.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -127,6 +129,10 @@ struct __align__(32) NodeRec {
 // node meta word
 #define GS_META_LEAF 0x80000000u
 #define GS_MAX_N ((1u << 27) - 1)  // a leaf count must fit the 27-bit field
+// a slot's node pool holds at most this many records (2^27 x 32 bytes =
+// 4 GiB), so the walk addresses them by 32-bit byte offsets; slots above 2^26
+// particles get this cap instead of 2N + 4096 records
+#define GS_NODE32_MAX (1u << 27)
 __host__ __device__ __forceinline__ uint32_t gs_meta_leaf(uint32_t depth, uint32_t count) {
   return GS_META_LEAF | (depth << 27) | count;
 }

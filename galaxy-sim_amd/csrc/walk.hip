@@ -140,13 +140,16 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
   int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;
   while (nd >= 0) {
-    const NodeRec rec = nodes[nd];  // one 32-byte scalar load
+    // one 32-byte scalar load at a 32-bit byte offset (a slot's node pool
+    // holds at most GS_NODE32_MAX = 2^27 records): the load takes the offset
+    // register directly, with no 64-bit address arithmetic on the scalar
+    // unit, which the walk keeps nearly as busy as the vector unit
+    const NodeRec rec = *(const NodeRec*)((const char*)nodes + ((uint32_t)nd << 5));
     const float4 c = rec.cm;
     const int4 inf = rec.info;
     const uint32_t meta = (uint32_t)inf.z;
     const uint32_t depth = gs_meta_depth(meta);
     const int32_t skip = inf.w;
-    // a massless node is skipped by every lane (nbody_bh.wgsl:86)
     // a massless node is skipped by every lane (nbody_bh.wgsl:86): its depth
     // test is made to fail on the scalar unit ((mass bits << 1) == 0)
     uint32_t mbits;
@@ -156,8 +159,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     const float size = __int_as_float(inf.x);
     const float s2 = size * size;
     const bool leaf = gs_meta_is_leaf(meta);
-    const uint32_t first = (uint32_t)inf.y;
-    const uint32_t count = leaf ? gs_meta_count(meta) : gs_meta_nc(meta);
+    const uint32_t first = (uint32_t)inf.y;  // leaf: first member; internal: last child
     const float rx = c.x - p.x, ry = c.y - p.y, rz = c.z - p.z;
     const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;  // exact: drives the decision
     // s2 < theta2*d2 || s2 < eps2/4  <=>  s2 < max(theta2*d2, eps2/4) (maxNum drops a
@@ -175,8 +177,8 @@ __global__ __launch_bounds__(1024) void walk_kernel(
         interact_fast(c.w, rx, ry, rz, d2, ax, ay, az, pot);
       if (COUNT) cnt++;
     }
-    int32_t next = skip;  // leaves and unopened nodes continue after the subtree
     if (leaf) {
+      const uint32_t count = gs_meta_count(meta);
       const bool own = act && i - first < count;
       if (own) dens = (float)count / (s2 * size);
       if (need) {  // lanes opening the leaf; the others are masked off
@@ -185,15 +187,21 @@ __global__ __launch_bounds__(1024) void walk_kernel(
         if (!EXACT && !COUNT && eps2 > 0.0f) {
           // eps > 0: the j == i term adds an exact zero to the acceleration
           // (r = 0, f finite), so members run without a per-pair index test
-          // and the self potential is taken back once per leaf
-          for (; j + 4 <= last; j += 4) {  // 4 members per batch of scalar loads
-            const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
+          // and the self potential is taken back once per leaf.  Members are
+          // addressed by 32-bit byte offsets (slot-local index < 2^27), so each
+          // batch of 4 scalar loads shares one offset register.
+          const char* pb = (const char*)posm;
+          uint32_t off = first << 4;
+          const uint32_t end = last << 4;
+          for (; off + 64 <= end; off += 64) {
+            const float4 q0 = *(const float4*)(pb + off), q1 = *(const float4*)(pb + off + 16);
+            const float4 q2 = *(const float4*)(pb + off + 32), q3 = *(const float4*)(pb + off + 48);
             pair_fast(q0, p, eps2, ax, ay, az, pot);
             pair_fast(q1, p, eps2, ax, ay, az, pot);
             pair_fast(q2, p, eps2, ax, ay, az, pot);
             pair_fast(q3, p, eps2, ax, ay, az, pot);
           }
-          for (; j < last; j++) pair_fast(posm[j], p, eps2, ax, ay, az, pot);
+          for (; off < end; off += 16) pair_fast(*(const float4*)(pb + off), p, eps2, ax, ay, az, pot);
           if (own) pot = pot + p.w * __builtin_amdgcn_rsqf(eps2);
         } else {
           for (; j + 4 <= last; j += 4) {
@@ -207,13 +215,24 @@ __global__ __launch_bounds__(1024) void walk_kernel(
             leaf_pair<EXACT, COUNT>(posm[j], j, i, p, G, eps2, ax, ay, az, pot, cnt);
         }
       }
-    } else if (need_mask) {
-      // descend to the last child (visited first): the opening lanes are
-      // active at the children's depth, the others only above this node
-      alim = need ? (int32_t)depth + 1 : min(alim, (int32_t)depth);
-      next = (int32_t)(first + count - 1);  // internal: firstChild + nChild - 1
     }
-    nd = next;
+    // depth limit: alim = min(alim, depth) + need.  On a descent the opening
+    // lanes (alim >= depth) become active at the children's depth and the
+    // others only above this node; after any other visit the next node is no
+    // deeper than this one, where the update changes no lane's activity, so
+    // it runs unconditionally (the need mask is the add's carry-in)
+    uint64_t carry;
+    asm("v_min_i32_e32 %0, %2, %0\n\tv_addc_co_u32_e64 %0, %1, 0, %0, %3"
+        : "+v"(alim), "=&s"(carry)
+        : "s"(depth), "s"(need_mask));
+    // an opened internal node continues at its last child (visited first);
+    // leaves (meta bit 31) and unopened nodes after the subtree: two scalar
+    // compare-selects
+    asm("s_cmp_eq_u64 %1, 0\n\ts_cselect_b32 %0, %2, %3\n\t"
+        "s_cmp_lt_i32 %4, 0\n\ts_cselect_b32 %0, %2, %0"
+        : "=&s"(nd)
+        : "s"(need_mask), "s"(skip), "s"(first), "s"(meta)
+        : "scc");
   }
   if (!EXACT) {
     ax *= G;
