@@ -103,40 +103,19 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <bool EXACT, bool STEP, bool COUNT>
-__global__ __launch_bounds__(1024) void walk_kernel(
-    const float4* __restrict__ posm_all, const float4* __restrict__ velp_all,
-    float4* __restrict__ posm_out_all, float4* __restrict__ velp_out_all,
-    float* __restrict__ energy_all, float4* __restrict__ acc_all, float* __restrict__ dens_all,
-    uint32_t* __restrict__ inter_all, const NodeRec* __restrict__ pool,
-    const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot,
-    const DevParams* __restrict__ prms, DevBatch* batch, double* __restrict__ red) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int waves = blockDim.x >> 6;
-
-  // a workgroup never spans two slots (slot ranges are GS_SLOT_ALIGN-aligned);
-  // blocks keep the hardware's round-robin spread over the 8 XCDs, so every
-  // XCD walks a mix of dense and sparse regions
-  const uint32_t blk = blockIdx.x;
-  const uint32_t slot = wblock_slot[blk];
-  const DevSlot sl = slots[slot];
-  const DevParams* __restrict__ prm = prms + slot;
-  const uint32_t n = sl.n;
-  const float4* __restrict__ posm = posm_all + sl.poff;
-  const NodeRec* __restrict__ nodes = pool + sl.noff;
-  const uint32_t i = (blk - sl.wblock0) * blockDim.x + threadIdx.x;  // slot-local particle
-  const bool valid = i < n;
-  if ((blk - sl.wblock0) * blockDim.x >= n) {  // a block of empty slot capacity
-    if (STEP && threadIdx.x < GS_NRED) red[(size_t)blk * GS_NRED + threadIdx.x] = 0.0;
-    return;
-  }
-  const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
-  const float qeps2 = 0.25f * eps2;
-
-  float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
-  uint32_t cnt = 0;
-
+// One particle per lane, the lanes of a wavefront walking one node sequence.
+// FASTLEAF (default path with eps > 0): leaf members run without the j == i
+// test, since that pair adds an exact zero to the acceleration (r = 0, f
+// finite); the self potential -m_i / eps is taken back once per leaf.  The
+// kernel picks the instance once per block, so the node loop carries no
+// test of it.
+template <bool EXACT, bool COUNT, bool FASTLEAF>
+__device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
+                                              const float4* __restrict__ posm, float4 p,
+                                              uint32_t i, bool valid, float G, float eps2,
+                                              float theta2, float qeps2, float& ax, float& ay,
+                                              float& az, float& pot, float& dens,
+                                              uint32_t& cnt) {
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
   int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;
   while (nd >= 0) {
@@ -184,7 +163,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(
       if (need) {  // lanes opening the leaf; the others are masked off
         const uint32_t last = first + count;
         uint32_t j = first;
-        if (!EXACT && !COUNT && eps2 > 0.0f) {
+        if (FASTLEAF) {
           // eps > 0: the j == i term adds an exact zero to the acceleration
           // (r = 0, f finite), so members run without a per-pair index test
           // and the self potential is taken back once per leaf.  Members are
@@ -201,7 +180,10 @@ __global__ __launch_bounds__(1024) void walk_kernel(
             pair_fast(q2, p, eps2, ax, ay, az, pot);
             pair_fast(q3, p, eps2, ax, ay, az, pot);
           }
-          for (; off < end; off += 16) pair_fast(*(const float4*)(pb + off), p, eps2, ax, ay, az, pot);
+          const uint32_t rem = end - off;  // 0-3 more members
+          if (rem >= 16) pair_fast(*(const float4*)(pb + off), p, eps2, ax, ay, az, pot);
+          if (rem >= 32) pair_fast(*(const float4*)(pb + off + 16), p, eps2, ax, ay, az, pot);
+          if (rem >= 48) pair_fast(*(const float4*)(pb + off + 32), p, eps2, ax, ay, az, pot);
           if (own) pot = pot + p.w * __builtin_amdgcn_rsqf(eps2);
         } else {
           for (; j + 4 <= last; j += 4) {
@@ -234,6 +216,47 @@ __global__ __launch_bounds__(1024) void walk_kernel(
         : "s"(need_mask), "s"(skip), "s"(first), "s"(meta)
         : "scc");
   }
+}
+
+template <bool EXACT, bool STEP, bool COUNT>
+__global__ __launch_bounds__(1024) void walk_kernel(
+    const float4* __restrict__ posm_all, const float4* __restrict__ velp_all,
+    float4* __restrict__ posm_out_all, float4* __restrict__ velp_out_all,
+    float* __restrict__ energy_all, float4* __restrict__ acc_all, float* __restrict__ dens_all,
+    uint32_t* __restrict__ inter_all, const NodeRec* __restrict__ pool,
+    const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot,
+    const DevParams* __restrict__ prms, DevBatch* batch, double* __restrict__ red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+
+  // a workgroup never spans two slots (slot ranges are GS_SLOT_ALIGN-aligned);
+  // blocks keep the hardware's round-robin spread over the 8 XCDs, so every
+  // XCD walks a mix of dense and sparse regions
+  const uint32_t blk = blockIdx.x;
+  const uint32_t slot = wblock_slot[blk];
+  const DevSlot sl = slots[slot];
+  const DevParams* __restrict__ prm = prms + slot;
+  const uint32_t n = sl.n;
+  const float4* __restrict__ posm = posm_all + sl.poff;
+  const NodeRec* __restrict__ nodes = pool + sl.noff;
+  const uint32_t i = (blk - sl.wblock0) * blockDim.x + threadIdx.x;  // slot-local particle
+  const bool valid = i < n;
+  if ((blk - sl.wblock0) * blockDim.x >= n) {  // a block of empty slot capacity
+    if (STEP && threadIdx.x < GS_NRED) red[(size_t)blk * GS_NRED + threadIdx.x] = 0.0;
+    return;
+  }
+  const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
+  const float qeps2 = 0.25f * eps2;
+
+  float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
+  uint32_t cnt = 0;
+  if (!EXACT && !COUNT && eps2 > 0.0f)
+    walk_particle<EXACT, COUNT, true>(nodes, posm, p, i, valid, G, eps2, theta2, qeps2, ax, ay,
+                                      az, pot, dens, cnt);
+  else
+    walk_particle<EXACT, COUNT, false>(nodes, posm, p, i, valid, G, eps2, theta2, qeps2, ax, ay,
+                                       az, pot, dens, cnt);
   if (!EXACT) {
     ax *= G;
     ay *= G;
