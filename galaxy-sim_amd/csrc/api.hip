@@ -536,7 +536,7 @@ int gs_create(const gs_config* cfg, gs_sim** out) {
   if (!cfg || !out) return GS_E_INVALID;
   *out = nullptr;
   if (cfg->n == 0 || cfg->n > GS_MAX_N) return GS_E_INVALID;
-  const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
+  const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 64;
   if (!valid_wg(wg)) return GS_E_INVALID;
   if (cfg->radix_bits != 0 && cfg->radix_bits != 8 && cfg->radix_bits != 10) return GS_E_INVALID;
   const uint32_t bucket = cfg->bucket ? cfg->bucket : 16;
@@ -566,7 +566,7 @@ int gs_create_pool(const gs_pool_config* cfg, gs_sim** out) {
   if (cfg->nslots == 0 || cfg->nslots > GS_MAX_SLOTS) return GS_E_INVALID;
   for (uint32_t k = 0; k < cfg->nslots; k++)
     if (cfg->capacity[k] == 0 || cfg->capacity[k] > GS_MAX_N) return GS_E_INVALID;
-  const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 256;
+  const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 64;
   if (!valid_wg(wg)) return GS_E_INVALID;
   if (cfg->radix_bits != 0 && cfg->radix_bits != 8 && cfg->radix_bits != 10) return GS_E_INVALID;
   gs_sim* s = nullptr;

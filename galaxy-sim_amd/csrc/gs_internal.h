@@ -119,7 +119,7 @@ struct DevBatch {
 };
 
 // leaves larger than this are refreshed by a whole wavefront each
-#define GS_BIG_LEAF 64
+#define GS_BIG_LEAF 32
 
 struct __align__(32) NodeRec {
   float4 cm;

@@ -91,16 +91,19 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
   }
 }
 
-// the queued large leaves, one wavefront each: 64 coalesced member loads per
-// chunk staged in LDS, then one lane adds them in index order (the same
-// sequence of roundings as the thread-per-leaf sum)
+// the queued large leaves, one wavefront each, 256 members per round: every
+// lane converts and multiplies 4 members (pm, x*pm, y*pm, z*pm in FP64, the
+// same roundings as the thread-per-leaf sum) into LDS, then lanes 0..3 each
+// carry one of the four running sums (m, cx, cy, cz) over the members in
+// index order, so the four sequential chains advance in one instruction
 __global__ __launch_bounds__(256) void refresh_big_kernel(const float4* __restrict__ posm,
                                                           NodeRec* __restrict__ pool,
                                                           const DevSlot* __restrict__ slots,
                                                           const DevBatch* __restrict__ batch,
                                                           const uint2* __restrict__ big,
                                                           uint32_t big_cap) {
-  __shared__ float4 s_mem[4][64];
+  constexpr uint32_t CH = 256;  // members per round
+  __shared__ double s_val[4][CH][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t nbig = min(batch->big_count, big_cap);
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
@@ -111,28 +114,39 @@ __global__ __launch_bounds__(256) void refresh_big_kernel(const float4* __restri
     const int4 inf = nodes[b.y].info;
     const uint32_t first = (uint32_t)inf.y, last = first + gs_meta_count((uint32_t)inf.z);
     const float4* pp = posm + sl.poff;
-    double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    for (uint32_t c = first; c < last; c += 64) {
-      const uint32_t j = c + (uint32_t)lane;
-      s_mem[w][lane] = j < last ? pp[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    double acc = 0.0;  // lane k < 4: sum k of (m, cx, cy, cz)
+    for (uint32_t c = first; c < last; c += CH) {
+#pragma unroll
+      for (uint32_t k = 0; k < CH / 64; k++) {
+        const uint32_t t = k * 64 + (uint32_t)lane;
+        const float4 q = c + t < last ? pp[c + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const double pm = (double)q.w;
+        s_val[w][t][0] = pm;
+        s_val[w][t][1] = __dmul_rn((double)q.x, pm);
+        s_val[w][t][2] = __dmul_rn((double)q.y, pm);
+        s_val[w][t][3] = __dmul_rn((double)q.z, pm);
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (lane == 0) {
-        const uint32_t nk = last - c < 64u ? last - c : 64u;
-        for (uint32_t t = 0; t < nk; t++) {
-          const float4 q = s_mem[w][t];
-          const double pm = (double)q.w;
-          m = __dadd_rn(m, pm);
-          cx = __dadd_rn(cx, __dmul_rn((double)q.x, pm));
-          cy = __dadd_rn(cy, __dmul_rn((double)q.y, pm));
-          cz = __dadd_rn(cz, __dmul_rn((double)q.z, pm));
+      if (lane < 4) {
+        const uint32_t nk = last - c < CH ? last - c : CH;
+        uint32_t t = 0;
+        for (; t + 8 <= nk; t += 8) {
+          double v[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) v[k] = s_val[w][t + k][lane];
+#pragma unroll
+          for (int k = 0; k < 8; k++) acc = __dadd_rn(acc, v[k]);
         }
+        for (; t < nk; t++) acc = __dadd_rn(acc, s_val[w][t][lane]);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    const double m = __shfl(acc, 0), cx = __shfl(acc, 1), cy = __shfl(acc, 2),
+                 cz = __shfl(acc, 3);
     if (lane == 0) store_com(nodes, b.y, m, cx, cy, cz);
   }
 }
@@ -175,8 +189,10 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
 }  // namespace
 
 hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
-  uint64_t g = (s->ntot / 4 + 255) / 256;  // nodes are ~0.1-0.3 of the pool
-  if (g > 8192) g = 8192;
+  // one thread per node, grid-stride: nodes are ~0.1-0.3 of the pool, and
+  // 2048 blocks of 256 fill the GPU
+  uint64_t g = (s->ntot / 4 + 255) / 256;
+  if (g > 2048) g = 2048;
   if (g < 64) g = 64;
   refresh_leaves_kernel<<<(uint32_t)g, 256, 0, s->stream>>>(posm, s->nodes, s->slots, s->sc,
                                                               s->nslots, s->batch, s->big_leaf,
