@@ -229,10 +229,20 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
 
-  // a workgroup never spans two slots (slot ranges are GS_SLOT_ALIGN-aligned);
-  // blocks keep the hardware's round-robin spread over the 8 XCDs, so every
-  // XCD walks a mix of dense and sparse regions
-  const uint32_t blk = blockIdx.x;
+  // a workgroup never spans two slots (slot ranges are GS_SLOT_ALIGN-aligned).
+  // The hardware deals workgroups to the 8 XCDs round-robin (XCD = id % 8);
+  // the work index is remapped so that each XCD takes runs of XCD_RUN
+  // consecutive blocks (neighbouring particles share most of their tree, and
+  // each XCD has its own L2), while the runs still interleave over the whole
+  // particle range, so every XCD walks a mix of dense and sparse regions.
+  // The remap is a permutation of the grid (a partial last group keeps the
+  // identity order).
+  constexpr uint32_t XCD_RUN = 64;
+  uint32_t blk = blockIdx.x;
+  if (blk < gridDim.x / (8 * XCD_RUN) * (8 * XCD_RUN)) {
+    const uint32_t r = blk % (8 * XCD_RUN);
+    blk = blk - r + (r % 8) * XCD_RUN + r / 8;
+  }
   const uint32_t slot = wblock_slot[blk];
   const DevSlot sl = slots[slot];
   const DevParams* __restrict__ prm = prms + slot;
