@@ -379,6 +379,8 @@ def main():
     ap.add_argument("--serve-per-class", type=int, default=8,
                     help="slots per power-of-two capacity class")
     args = ap.parse_args()
+    if args.lib:
+        os.environ["GALAXY_HIP_LIB"] = os.path.abspath(args.lib)
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -390,8 +392,6 @@ def main():
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
-    if args.lib:
-        os.environ["GALAXY_HIP_LIB"] = args.lib
     if args.only == "batch":
         print(json.dumps(bench_batch(args)))
         return

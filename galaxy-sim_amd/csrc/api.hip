@@ -144,6 +144,7 @@ void free_all(gs_sim* s) {
   f(s->sort_status);
   f(s->bbox_part);
   f(s->nodes);
+  f(s->nsize);
   f(s->energy);
   f(s->acc);
   f(s->dens);
@@ -388,6 +389,7 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     dalloc(&s->sort_status, (size_t)GS_HIST_WORDS * s->sort_tiles, "sort status");
     dalloc(&s->bbox_part, (size_t)s->chunks * 8, "bbox");
     dalloc(&s->nodes, s->ntot, "nodes");
+    dalloc(&s->nsize, s->ntot, "node sizes");
     dalloc(&s->energy, P, "energy");
     dalloc(&s->acc, P, "acc");
     dalloc(&s->dens, P, "density");
@@ -449,6 +451,14 @@ void download_slot(gs_sim* s, uint32_t slot, void* dst, const void* dev_base, si
                        hipMemcpyDeviceToHost, s->stream),
         what);
   sync(s);
+}
+
+// f32(size * size), the value a node record holds
+float host_s2(float size) {
+  const int b = gs_s2_bits(size);
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
 }
 
 DevScalars read_scalars(gs_sim* s, uint32_t slot) {
@@ -706,8 +716,12 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
     // host validation: every index the device will follow stays in bounds;
     // the walk's (skip, depth) links follow from the BFS numbering
     std::vector<int2> links(num_nodes, make_int2(-1, 0));
+    float min_s2 = INFINITY;  // the walk's eps^2/4 opening test needs it
     for (uint64_t i = 0; i < num_nodes; i++) {
       const uint32_t* w = nodes + i * 16;
+      float size;
+      memcpy(&size, &w[4], 4);
+      min_s2 = fminf(min_s2, host_s2(size));
       if (w[5] == 1u) {
         need(w[6] <= h.d.n && w[7] <= h.d.n - w[6], GS_E_INVALID, "leaf range outside [0, N)");
       } else {
@@ -731,6 +745,7 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
     DevScalars sc = read_scalars(s, 0);
     sc.num_nodes = (uint32_t)num_nodes;
     sc.num_levels = num_levels;
+    sc.min_s2 = min_s2;
     for (uint32_t k = 0; k <= GS_MAX_LEVELS; k++) {
       sc.level_start[k] = k < num_levels ? level_ranges[2 * k] : 0;
       sc.level_count[k] = k < num_levels ? level_ranges[2 * k + 1] : 0;
@@ -1108,10 +1123,13 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     // root segment and per-level sizes from the caller's rootSize
     DevScalars sc = read_scalars(s, 0);
     double sz = root_size;
+    float min_s2 = INFINITY;
     for (int d = 0; d <= GS_MAX_LEVELS; d++) {
       sc.level_size[d] = (float)sz;
+      min_s2 = fminf(min_s2, host_s2(sc.level_size[d]));
       sz *= 0.5;
     }
+    sc.min_s2 = min_s2;
     memcpy(&sc.size_bits, &root_size, 8);
     for (int d = 0; d <= GS_MAX_LEVELS; d++) sc.level_start[d] = sc.level_count[d] = 0;
     sc.level_count[0] = 1;
@@ -1121,12 +1139,13 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     DevBatch b;
     memset(&b, 0, sizeof b);
     check(hipMemcpyAsync(s->batch, &b, sizeof b, hipMemcpyHostToDevice, s->stream), "batch");
-    int4 root = make_int4(0, 0, (int)gs_meta_open(0, n), -1);
-    float f0 = sc.level_size[0];
-    memcpy(&root.x, &f0, 4);
+    const float f0 = sc.level_size[0];
+    const int4 root = make_int4(gs_s2_bits(f0), 0, (int)gs_meta_open(0, n), -1);
     check(hipMemcpyAsync(&s->nodes[h.d.noff].info, &root, sizeof root, hipMemcpyHostToDevice,
                          s->stream),
           "root");
+    check(hipMemcpyAsync(&s->nsize[h.d.noff], &f0, sizeof f0, hipMemcpyHostToDevice, s->stream),
+          "root size");
     check(gs_launch_build(s), "build");
     check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
     h.have_tree = true;

@@ -36,7 +36,7 @@ __device__ __forceinline__ void st_agent64(uint64_t* p, uint64_t v) {
 }
 
 __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
-    const uint32_t* __restrict__ keys, NodeRec* __restrict__ pool,
+    const uint32_t* __restrict__ keys, NodeRec* __restrict__ pool, float* __restrict__ nsize_pool,
     const DevSlot* __restrict__ slots, DevScalars* scs, DevBatch* batch, uint32_t nslots, int d,
     uint64_t* status, uint64_t* status_clear, uint32_t clear_n) {
   __shared__ uint32_t s_tile;
@@ -78,6 +78,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const DevSlot sl = slots[slot];
     DevScalars* sc = scs + slot;
     NodeRec* nodes = pool + sl.noff;
+    float* nsize = nsize_pool + sl.noff;
     const uint32_t* codes = keys + sl.poff;
     const uint32_t n = sl.n, bucket = sl.bucket, cap = sl.ncap;
     const uint64_t max_nodes = sl.max_nodes;
@@ -85,8 +86,10 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const uint32_t count = sc->level_count[d];
     const uint32_t start = sc->level_start[d];
     const uint32_t next_start = start + count;
-    const uint32_t size_d = __float_as_uint(sc->level_size[d]);
-    const uint32_t size_c = __float_as_uint(sc->level_size[d + 1]);
+    // records hold size * size; the children's sizes also go to nsize
+    const int s2_d = gs_s2_bits(sc->level_size[d]);
+    const float size_c = sc->level_size[d + 1];
+    const int s2_c = gs_s2_bits(size_c);
 
     // phase 1: octant boundaries b_0..b_7 (+ hi), 8 lanes per node; each
     // thread owns one octant of R nodes and advances their R binary searches
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
       const int4 inf = nodes[idx].info;
       const int32_t skip = inf.w;
       if (leaf) {
-        nodes[idx].info = make_int4((int)size_d, inf.y,
+        nodes[idx].info = make_int4(s2_d, inf.y,
                                      (int)gs_meta_leaf((uint32_t)d, gs_meta_count((uint32_t)inf.z)),
                                      skip);
       } else {
@@ -247,14 +250,15 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
             const uint32_t b1 = k < 7 ? s_b[threadIdx.x][k + 1] : hi;
             if (c < cap) {
               // pre-order with octant 7 first: after child c comes c - 1
-              nodes[c].info = make_int4((int)size_c, (int)b0,
+              nsize[c] = size_c;
+              nodes[c].info = make_int4(s2_c, (int)b0,
                                          (int)gs_meta_open((uint32_t)d + 1, b1 - b0),
                                          c > fc ? (int32_t)c - 1 : skip);
             }
             c++;
           }
         }
-        nodes[idx].info = make_int4((int)size_d, (int)(fc + nchild - 1),  // last child
+        nodes[idx].info = make_int4(s2_d, (int)(fc + nchild - 1),  // last child
                                      (int)gs_meta_internal((uint32_t)d, mask, nchild), skip);
       }
     }
@@ -269,7 +273,7 @@ hipError_t gs_launch_build(gs_sim* s) {
   if (grid < 1) grid = 1;
   for (int d = 0; d <= GS_MAX_LEVEL; d++) {
     build_level_kernel<<<grid, GS_BUILD_TILE, 0, s->stream>>>(
-        s->keys[gs_sorted_buf(s)], s->nodes, s->slots, s->sc, s->batch, s->nslots, d,
+        s->keys[gs_sorted_buf(s)], s->nodes, s->nsize, s->slots, s->sc, s->batch, s->nslots, d,
         s->build_status[d & 1],
         s->build_status[(d + 1) & 1], s->build_words);
     hipError_t e = hipGetLastError();
@@ -281,6 +285,7 @@ hipError_t gs_launch_build(gs_sim* s) {
 // ---- expansion to the reference's 64-byte node layout (mortonTree.js:12-21)
 namespace {
 __global__ __launch_bounds__(256) void node64_kernel(const NodeRec* __restrict__ nodes,
+                                                     const float* __restrict__ nsize,
                                                      const DevScalars* __restrict__ sc,
                                                      uint32_t* __restrict__ out) {
   const uint32_t num = sc->num_nodes;
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(256) void node64_kernel(const NodeRec* __restrict__
     w[1] = __float_as_uint(c.y);
     w[2] = __float_as_uint(c.z);
     w[3] = __float_as_uint(c.w);
-    w[4] = (uint32_t)inf.x;
+    w[4] = __float_as_uint(nsize[i]);
     const uint32_t meta = (uint32_t)inf.z;
     const bool leaf = gs_meta_is_leaf(meta);
     w[5] = leaf ? 1u : 0u;
@@ -317,6 +322,7 @@ __global__ __launch_bounds__(256) void node64_kernel(const NodeRec* __restrict__
 
 // ---- import of a tree in the reference's 64-byte layout (uploadBHTree) ----
 __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nodes,
+                                                       float* __restrict__ nsize,
                                                        DevBatch* batch,
                                                        const uint32_t* __restrict__ in,
                                                        const int2* __restrict__ links,
@@ -328,8 +334,11 @@ __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nod
                               __uint_as_float(a.w));
     const int32_t skip = links[i].x;
     const uint32_t depth = (uint32_t)links[i].y;
+    const float size = __uint_as_float(b.x);
+    const int s2 = gs_s2_bits(size);
+    nsize[i] = size;
     if (b.y == 1u) {
-      nodes[i].info = make_int4((int)b.x, (int)b.z, (int)gs_meta_leaf(depth, b.w), skip);
+      nodes[i].info = make_int4(s2, (int)b.z, (int)gs_meta_leaf(depth, b.w), skip);
       continue;
     }
     const uint32_t ch[8] = {c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
@@ -344,7 +353,7 @@ __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nod
       }
     }
     // internal: y = last child (a childless node continues at its skip)
-    nodes[i].info = make_int4((int)b.x, nc ? (int)(fc + nc - 1) : skip,
+    nodes[i].info = make_int4(s2, nc ? (int)(fc + nc - 1) : skip,
                               (int)gs_meta_internal(depth, mask, nc), skip);
   }
 }
@@ -381,7 +390,8 @@ hipError_t gs_launch_node64(gs_sim* s, uint32_t slot) {
   const HostSlot& h = s->hs[slot];
   uint32_t grid = (h.d.ncap + 255) / 256;
   if (grid > 4096) grid = 4096;
-  node64_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + h.d.noff, s->sc + slot, s->node64);
+  node64_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + h.d.noff, s->nsize + h.d.noff,
+                                              s->sc + slot, s->node64);
   return hipGetLastError();
 }
 
@@ -390,7 +400,8 @@ hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, c
   uint32_t grid = (num + 255) / 256;
   if (grid > 4096) grid = 4096;
   if (grid < 1) grid = 1;
-  import64_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + s->hs[slot].d.noff, s->batch, words,
+  import64_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + s->hs[slot].d.noff,
+                                                s->nsize + s->hs[slot].d.noff, s->batch, words,
                                                 links, num);
   return hipGetLastError();
 }

@@ -113,7 +113,8 @@ __global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ po
 
 __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ part,
                                                    const DevSlot* __restrict__ slots,
-                                                   DevScalars* scs, NodeRec* nodes) {
+                                                   DevScalars* scs, NodeRec* nodes,
+                                                   float* nsize) {
   const uint32_t slot = blockIdx.x;
   const DevSlot sl = slots[slot];
   DevScalars* sc = scs + slot;
@@ -170,10 +171,14 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   sc->size_bits = size;
   // node sizes per depth: f32(rootSize * 0.5^d) (mortonTree.js:124,161)
   uint64_t sz = size;
+  float min_s2 = INFINITY;
   for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
-    sc->level_size[dpt] = __uint_as_float(sd::to_f32(sz));
+    const float f = __uint_as_float(sd::to_f32(sz));
+    sc->level_size[dpt] = f;
+    min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));  // fminf drops a NaN
     if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
   }
+  sc->min_s2 = min_s2;
   // level 0 = the root, covering [0, n); an empty slot has no tree
   for (int k = 0; k <= GS_MAX_LEVELS; k++) {
     sc->level_start[k] = 0;
@@ -182,9 +187,11 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   sc->level_count[0] = sl.n ? 1u : 0u;
   sc->num_nodes = sl.n ? 1u : 0u;
   sc->num_levels = 0;
-  if (sl.n)
-    nodes[sl.noff].info = make_int4((int)__float_as_uint(sc->level_size[0]), 0,
+  if (sl.n) {
+    nsize[sl.noff] = sc->level_size[0];
+    nodes[sl.noff].info = make_int4(gs_s2_bits(sc->level_size[0]), 0,
                                     (int)gs_meta_open(0, sl.n), -1);
+  }
 }
 
 __device__ __forceinline__ uint32_t quant(float p, uint64_t mn, uint64_t s) {
@@ -310,7 +317,8 @@ hipError_t gs_launch_bbox(gs_sim* s, const float4* posm) {
 }
 
 hipError_t gs_launch_root(gs_sim* s) {
-  root_kernel<<<s->nslots, 256, 0, s->stream>>>(s->bbox_part, s->slots, s->sc, s->nodes);
+  root_kernel<<<s->nslots, 256, 0, s->stream>>>(s->bbox_part, s->slots, s->sc, s->nodes,
+                                                 s->nsize);
   return hipGetLastError();
 }
 

@@ -17,13 +17,16 @@
 //   keys[2][P], vals[2][P]   u32         radix-sort ping-pong (code, index)
 //   node[C]     32-byte record (one s_load_dwordx8 in the walk):
 //               float4 cm   (comx, comy, comz, mass)
-//               int4   info (size bits, first | lastChild, meta, skip)
+//               int4   info (size*size bits, first | lastChild, meta, skip)
 //               meta: bit 31 leaf, bits 27..30 depth, then
 //                     leaf: bits 0..26 particle count
 //                     internal: bits 0..3 child count, bits 8..15 octant mask
 //               skip: the node that follows this node's subtree in the
 //                     walk's pre-order (previous sibling, else the parent's
 //                     skip; -1 ends the walk)
+//   nsize[C]    float node size (the record holds its square, which is what the
+//               walk's opening test reads; the size itself is read only for
+//               the density of a particle's own leaf and for downloads)
 // Node and particle indices inside a slot are slot-local (0-based).  A slot's
 // node pool holds min(2 cap + 4096, 2^27) nodes (mortonTree.js:88).  Children
 // of a BFS node are contiguous, so an internal node stores (lastChild, nChild,
@@ -102,7 +105,7 @@ struct DevScalars {
   uint32_t nfinite;      // particles that entered the bbox
   uint32_t num_nodes;
   uint32_t num_levels;
-  uint32_t pad_;
+  float min_s2;          // smallest size * size over the tree's nodes (NaN ignored)
   uint32_t level_start[GS_MAX_LEVELS + 1];
   uint32_t level_count[GS_MAX_LEVELS + 1];
   float level_size[GS_MAX_LEVELS + 1];
@@ -125,6 +128,18 @@ struct __align__(32) NodeRec {
   float4 cm;
   int4 info;
 };
+
+// the record's first info word: f32(size * size) as the walk's opening test
+// evaluates it (nbody_bh.wgsl:99), one rounding
+__host__ __device__ __forceinline__ int gs_s2_bits(float size) {
+  const float s2 = size * size;
+  union {
+    float f;
+    int i;
+  } u;
+  u.f = s2;
+  return u.i;
+}
 
 // node meta word
 #define GS_META_LEAF 0x80000000u
@@ -233,6 +248,7 @@ struct gs_sim {
   uint64_t* build_status[2] = {nullptr, nullptr};
   float* bbox_part = nullptr;       // chunks x 8
   NodeRec* nodes = nullptr;         // ntot records
+  float* nsize = nullptr;           // ntot node sizes
   float* energy = nullptr;
   float4* acc = nullptr;
   float* dens = nullptr;

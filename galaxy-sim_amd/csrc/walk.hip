@@ -20,7 +20,9 @@
 // equal the reference walk's.  The node index is
 // wave-uniform, so node records and leaf members are scalar (SMEM) broadcast
 // loads, there is no stack in memory, and divergence costs only the lanes
-// masked off.
+// masked off.  The active, monopole and opening lanes are 64-bit wave masks
+// (two vector compares per node, combined on the scalar unit) that set the
+// execution mask of the monopole and leaf branches directly.
 //
 // Arithmetic: the opening test uses the exact reference expression order
 // d2 = ((rx*rx + ry*ry) + rz*rz) + eps2 with no contraction (the library is
@@ -109,14 +111,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 // finite); the self potential -m_i / eps is taken back once per leaf.  The
 // kernel picks the instance once per block, so the node loop carries no
 // test of it.
-template <bool EXACT, bool COUNT, bool FASTLEAF>
+template <bool EXACT, bool COUNT, bool FASTLEAF, bool QCHECK>
 __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
+                                              const float* __restrict__ nsize,
                                               const float4* __restrict__ posm, float4 p,
-                                              uint32_t i, bool valid, float G, float eps2,
-                                              float theta2, float qeps2, float& ax, float& ay,
-                                              float& az, float& pot, float& dens,
-                                              uint32_t& cnt) {
+                                              uint32_t i, bool valid, float G,
+                                              float eps2, float theta2, uint32_t qbits,
+                                              float& ax, float& ay, float& az, float& pot,
+                                              float& dens, uint32_t& cnt) {
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
+  const float selfpot = FASTLEAF ? p.w * __builtin_amdgcn_rsqf(eps2) : 0.f;
   int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;
   while (nd >= 0) {
     // one 32-byte scalar load at a 32-bit byte offset (a slot's node pool
@@ -134,33 +138,43 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
     uint32_t mbits;
     asm("s_lshl_b32 %0, %1, 1" : "=s"(mbits) : "s"(__float_as_uint(c.w)));
     const int32_t dtest = mbits != 0u ? (int32_t)depth : 64;
-    const bool act = dtest <= alim;
-    const float size = __int_as_float(inf.x);
-    const float s2 = size * size;
+    // the lanes that opened every ancestor, as a wave mask (one compare)
+    const uint64_t act = __builtin_amdgcn_ballot_w64(dtest <= alim);
+    const float s2 = __int_as_float(inf.x);  // the record holds size * size
     const bool leaf = gs_meta_is_leaf(meta);
     const uint32_t first = (uint32_t)inf.y;  // leaf: first member; internal: last child
     const float rx = c.x - p.x, ry = c.y - p.y, rz = c.z - p.z;
     const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;  // exact: drives the decision
-    // s2 < theta2*d2 || s2 < eps2/4  <=>  s2 < max(theta2*d2, eps2/4) (maxNum drops a
-    // NaN product, as the || does); inactive lanes compare against NaN, so both
-    // the monopole test and its complement are false for them
-    const float lim = act ? fmaxf(theta2 * d2, qeps2) : __builtin_nanf("");
-    const bool mono = s2 < lim;
-    const bool need = s2 >= lim;  // == act && !mono: s2 and an active lim are never NaN
-    // the same compare as a wave lane mask (ordered >=: the v_cmp result itself)
-    const uint64_t need_mask = __builtin_amdgcn_fcmpf(s2, lim, 3 /* OGE */);
-    if (mono) {
+    // monopole iff s2 < theta2*d2 || s2 < eps2/4 (nbody_bh.wgsl:97-112).  The
+    // first test is one compare per lane (false for a NaN d2, as in the
+    // reference); the second depends on the node alone and is an integer
+    // compare of non-negative float bits on the scalar unit, made only when
+    // some node of the tree is that small (QCHECK).  Every other active lane
+    // opens the node.
+    uint64_t mono_raw = __builtin_amdgcn_ballot_w64(s2 < theta2 * d2);
+    if (QCHECK && (uint32_t)inf.x < qbits) mono_raw = ~0ull;
+    const uint64_t mono_mask = act & mono_raw;
+    const uint64_t need_mask = act & ~mono_raw;
+    if (__builtin_amdgcn_inverse_ballot_w64(mono_mask)) {
       if (EXACT)
         interact_exact(G * c.w, rx, ry, rz, d2, ax, ay, az, pot);
       else
         interact_fast(c.w, rx, ry, rz, d2, ax, ay, az, pot);
       if (COUNT) cnt++;
     }
+    // lanes that descend into the node's children (none for a leaf)
+    uint64_t desc_mask = need_mask;
     if (leaf) {
+      desc_mask = 0;
       const uint32_t count = gs_meta_count(meta);
-      const bool own = act && i - first < count;
-      if (own) dens = (float)count / (s2 * size);
-      if (need) {  // lanes opening the leaf; the others are masked off
+      // the leaf holding a lane's own particle sets its density.  FASTLEAF: a
+      // lane that opens its own leaf takes back the self potential that the
+      // member loop below adds.
+      if (__builtin_amdgcn_inverse_ballot_w64(act) && i - first < count) {
+        dens = (float)count / (s2 * nsize[nd]);
+        if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
+      }
+      if (__builtin_amdgcn_inverse_ballot_w64(need_mask)) {  // lanes opening the leaf
         const uint32_t last = first + count;
         uint32_t j = first;
         if (FASTLEAF) {
@@ -184,7 +198,6 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
           if (rem >= 16) pair_fast(*(const float4*)(pb + off), p, eps2, ax, ay, az, pot);
           if (rem >= 32) pair_fast(*(const float4*)(pb + off + 16), p, eps2, ax, ay, az, pot);
           if (rem >= 48) pair_fast(*(const float4*)(pb + off + 32), p, eps2, ax, ay, az, pot);
-          if (own) pot = pot + p.w * __builtin_amdgcn_rsqf(eps2);
         } else {
           for (; j + 4 <= last; j += 4) {
             const float4 q0 = posm[j], q1 = posm[j + 1], q2 = posm[j + 2], q3 = posm[j + 3];
@@ -198,22 +211,20 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
         }
       }
     }
-    // depth limit: alim = min(alim, depth) + need.  On a descent the opening
+    // depth limit: alim = min(alim, depth) + desc.  On a descent the opening
     // lanes (alim >= depth) become active at the children's depth and the
     // others only above this node; after any other visit the next node is no
     // deeper than this one, where the update changes no lane's activity, so
-    // it runs unconditionally (the need mask is the add's carry-in)
+    // it runs unconditionally (the descent mask is the add's carry-in)
     uint64_t carry;
     asm("v_min_i32_e32 %0, %2, %0\n\tv_addc_co_u32_e64 %0, %1, 0, %0, %3"
         : "+v"(alim), "=&s"(carry)
-        : "s"(depth), "s"(need_mask));
+        : "s"(depth), "s"(desc_mask));
     // an opened internal node continues at its last child (visited first);
-    // leaves (meta bit 31) and unopened nodes after the subtree: two scalar
-    // compare-selects
-    asm("s_cmp_eq_u64 %1, 0\n\ts_cselect_b32 %0, %2, %3\n\t"
-        "s_cmp_lt_i32 %4, 0\n\ts_cselect_b32 %0, %2, %0"
-        : "=&s"(nd)
-        : "s"(need_mask), "s"(skip), "s"(first), "s"(meta)
+    // leaves and unopened nodes after the subtree
+    asm("s_cmp_eq_u64 %1, 0\n\ts_cselect_b32 %0, %2, %3"
+        : "=s"(nd)
+        : "s"(desc_mask), "s"(skip), "s"(first)
         : "scc");
   }
 }
@@ -224,8 +235,9 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     float4* __restrict__ posm_out_all, float4* __restrict__ velp_out_all,
     float* __restrict__ energy_all, float4* __restrict__ acc_all, float* __restrict__ dens_all,
     uint32_t* __restrict__ inter_all, const NodeRec* __restrict__ pool,
-    const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot,
-    const DevParams* __restrict__ prms, DevBatch* batch, double* __restrict__ red) {
+    const float* __restrict__ nsize_all, const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot,
+    const DevParams* __restrict__ prms, const DevScalars* __restrict__ scs, DevBatch* batch,
+    double* __restrict__ red) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
 
@@ -257,16 +269,28 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   }
   const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
+  // eps2/4 as float bits for the scalar unit's compare (none when NaN)
   const float qeps2 = 0.25f * eps2;
+  const uint32_t qbits = qeps2 == qeps2 ? __float_as_uint(qeps2) : 0u;
+  const float* __restrict__ nsize = nsize_all + sl.noff;
+
+  // the eps^2/4 test can only accept a node if some node of the tree has
+  // size^2 below eps^2/4 (never at depth <= 10 for the usual eps)
+  const bool qcheck = scs[slot].min_s2 < qeps2;
 
   float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
   uint32_t cnt = 0;
-  if (!EXACT && !COUNT && eps2 > 0.0f)
-    walk_particle<EXACT, COUNT, true>(nodes, posm, p, i, valid, G, eps2, theta2, qeps2, ax, ay,
-                                      az, pot, dens, cnt);
-  else
-    walk_particle<EXACT, COUNT, false>(nodes, posm, p, i, valid, G, eps2, theta2, qeps2, ax, ay,
-                                       az, pot, dens, cnt);
+#define GS_WALK(FL, QC)                                                                      \
+  walk_particle<EXACT, COUNT, FL, QC>(nodes, nsize, posm, p, i, valid, G, eps2, theta2,      \
+                                      qbits, ax, ay, az, pot, dens, cnt)
+  if (!EXACT && !COUNT && eps2 > 0.0f) {
+    if (qcheck) GS_WALK(true, true);
+    else GS_WALK(true, false);
+  } else {
+    if (qcheck) GS_WALK(false, true);
+    else GS_WALK(false, false);
+  }
+#undef GS_WALK
   if (!EXACT) {
     ax *= G;
     ay *= G;
@@ -376,7 +400,8 @@ hipError_t launch(gs_sim* s) {
   const int c = s->cur;
   walk_kernel<EXACT, STEP, COUNT><<<s->walk_blocks, s->walk_wg, 0, s->stream>>>(
       s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
-      s->inter, s->nodes, s->slots, s->wblock_slot, s->prm, s->batch, s->red_part);
+      s->inter, s->nodes, s->nsize, s->slots, s->wblock_slot, s->prm, s->sc, s->batch,
+      s->red_part);
   return hipGetLastError();
 }
 
