@@ -145,6 +145,7 @@ void free_all(gs_sim* s) {
   f(s->bbox_part);
   f(s->nodes);
   f(s->nsize);
+  f(s->massless);
   f(s->energy);
   f(s->acc);
   f(s->dens);
@@ -390,6 +391,7 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     dalloc(&s->bbox_part, (size_t)s->chunks * 8, "bbox");
     dalloc(&s->nodes, s->ntot, "nodes");
     dalloc(&s->nsize, s->ntot, "node sizes");
+    dalloc(&s->massless, nslots, "massless flags");
     dalloc(&s->energy, P, "energy");
     dalloc(&s->acc, P, "acc");
     dalloc(&s->dens, P, "density");
@@ -410,6 +412,7 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     check(gs_launch_fill_u32((uint32_t*)s->sc, 0, sizeof(DevScalars) / 4 * nslots, s->stream),
           "fill");
     check(gs_launch_fill_u32((uint32_t*)s->batch, 0, sizeof(DevBatch) / 4, s->stream), "fill");
+    check(gs_launch_fill_u32(s->massless, 1u, nslots, s->stream), "fill");
     check(gs_launch_fill_u32((uint32_t*)s->energy, 0, P, s->stream), "fill");
     for (int k = 0; k < 2; k++) {
       check(gs_launch_fill_u32((uint32_t*)s->build_status[k], 0, 2ull * s->build_words,
@@ -717,11 +720,14 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
     // the walk's (skip, depth) links follow from the BFS numbering
     std::vector<int2> links(num_nodes, make_int2(-1, 0));
     float min_s2 = INFINITY;  // the walk's eps^2/4 opening test needs it
+    uint32_t massless = 0;    // and its massless-node test this
     for (uint64_t i = 0; i < num_nodes; i++) {
       const uint32_t* w = nodes + i * 16;
-      float size;
+      float size, mass;
       memcpy(&size, &w[4], 4);
+      memcpy(&mass, &w[3], 4);
       min_s2 = fminf(min_s2, host_s2(size));
+      if (mass == 0.0f) massless = 1;
       if (w[5] == 1u) {
         need(w[6] <= h.d.n && w[7] <= h.d.n - w[6], GS_E_INVALID, "leaf range outside [0, N)");
       } else {
@@ -751,6 +757,7 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
       sc.level_count[k] = k < num_levels ? level_ranges[2 * k + 1] : 0;
     }
     check(hipMemcpyAsync(s->sc, &sc, sizeof sc, hipMemcpyHostToDevice, s->stream), "scalars");
+    check(hipMemcpyAsync(s->massless, &massless, 4, hipMemcpyHostToDevice, s->stream), "flag");
     int2* dlinks = nullptr;
     dalloc(&dlinks, num_nodes, "links");
     hipError_t e = hipMemcpyAsync(dlinks, links.data(), sizeof(int2) * num_nodes,

@@ -249,6 +249,9 @@ struct gs_sim {
   float* bbox_part = nullptr;       // chunks x 8
   NodeRec* nodes = nullptr;         // ntot records
   float* nsize = nullptr;           // ntot node sizes
+  // per slot: nonzero when the tree may hold a node of zero mass (set by the
+  // refresh and by tree uploads; a fresh handle starts at 1)
+  uint32_t* massless = nullptr;
   float* energy = nullptr;
   float4* acc = nullptr;
   float* dens = nullptr;

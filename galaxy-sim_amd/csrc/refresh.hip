@@ -24,8 +24,11 @@ namespace {
 
 constexpr int LEAF_BATCH = 8;
 
+// a node stored with zero mass marks its slot, so that the walk runs its
+// variant with the massless-node test (nbody_bh.wgsl:86)
 __device__ __forceinline__ void store_com(NodeRec* nodes, uint32_t idx, double m, double cx,
-                                          double cy, double cz) {
+                                          double cy, double cz, uint32_t* massless) {
+  if ((float)m == 0.0f) *massless = 1u;  // the stored mass, after rounding
   const double inv = m > 0.0 ? __ddiv_rn(1.0, m) : 0.0;
   nodes[idx].cm = make_float4((float)__dmul_rn(cx, inv), (float)__dmul_rn(cy, inv),
                               (float)__dmul_rn(cz, inv), (float)m);
@@ -64,7 +67,8 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
                                                              const DevScalars* __restrict__ scs,
                                                              uint32_t nslots, DevBatch* batch,
                                                              uint2* __restrict__ big,
-                                                             uint32_t big_cap) {
+                                                             uint32_t big_cap,
+                                                             uint32_t* __restrict__ massless) {
   __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1 + GS_MAX_SLOTS / 64];
   gs_slot_prefix(threadIdx.x < nslots ? scs[threadIdx.x].num_nodes : 0u, nslots, s_pre);
   const uint32_t total = s_pre[nslots];
@@ -87,7 +91,7 @@ __global__ __launch_bounds__(256) void refresh_leaves_kernel(const float4* __res
     }
     double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     sum_members(posm + sl.poff, first, first + count, m, cx, cy, cz);
-    store_com(nodes, idx, m, cx, cy, cz);
+    store_com(nodes, idx, m, cx, cy, cz, massless + slot);
   }
 }
 
@@ -101,7 +105,8 @@ __global__ __launch_bounds__(256) void refresh_big_kernel(const float4* __restri
                                                           const DevSlot* __restrict__ slots,
                                                           const DevBatch* __restrict__ batch,
                                                           const uint2* __restrict__ big,
-                                                          uint32_t big_cap) {
+                                                          uint32_t big_cap,
+                                                          uint32_t* __restrict__ massless) {
   constexpr uint32_t CH = 256;  // members per round
   __shared__ double s_val[4][CH][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(256) void refresh_big_kernel(const float4* __restri
     }
     const double m = __shfl(acc, 0), cx = __shfl(acc, 1), cy = __shfl(acc, 2),
                  cz = __shfl(acc, 3);
-    if (lane == 0) store_com(nodes, b.y, m, cx, cy, cz);
+    if (lane == 0) store_com(nodes, b.y, m, cx, cy, cz, massless + b.x);
   }
 }
 
@@ -157,7 +162,8 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
                                                             const DevSlot* __restrict__ slots,
                                                             const DevScalars* __restrict__ scs,
                                                             uint32_t nslots, int d,
-                                                            DevBatch* batch) {
+                                                            DevBatch* batch,
+                                                            uint32_t* __restrict__ massless) {
   __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1 + GS_MAX_SLOTS / 64];
   // the large-leaf queue was consumed by the previous launch: empty it for
   // the next refresh
@@ -182,7 +188,7 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
       cy = __dadd_rn(cy, __dmul_rn((double)q.y, qm));
       cz = __dadd_rn(cz, __dmul_rn((double)q.z, qm));
     }
-    store_com(nodes, idx, m, cx, cy, cz);
+    store_com(nodes, idx, m, cx, cy, cz, massless + slot);
   }
 }
 
@@ -194,13 +200,15 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
   uint64_t g = (s->ntot / 4 + 255) / 256;
   if (g > 2048) g = 2048;
   if (g < 64) g = 64;
+  hipError_t e = gs_launch_fill_u32(s->massless, 0u, s->nslots, s->stream);
+  if (e != hipSuccess) return e;
   refresh_leaves_kernel<<<(uint32_t)g, 256, 0, s->stream>>>(posm, s->nodes, s->slots, s->sc,
                                                               s->nslots, s->batch, s->big_leaf,
-                                                              s->big_cap);
-  hipError_t e = hipGetLastError();
+                                                              s->big_cap, s->massless);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   refresh_big_kernel<<<256, 256, 0, s->stream>>>(posm, s->nodes, s->slots, s->batch, s->big_leaf,
-                                                 s->big_cap);
+                                                 s->big_cap, s->massless);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   // internal nodes exist only at depths 0..9; a level's node count is bounded
@@ -211,7 +219,7 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
     uint32_t gl = (uint32_t)((bound + 255) / 256);
     if (gl > 2048) gl = 2048;
     refresh_level_kernel<<<gl, 256, 0, s->stream>>>(s->nodes, s->slots, s->sc, s->nslots, d,
-                                                     s->batch);
+                                                     s->batch, s->massless);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -42,6 +42,8 @@
 
 namespace {
 
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+
 // EXACT: the reference expression f = G*m / (d2*sqrt(d2)), IEEE sqrt and '/',
 // separate multiply and add (bit-identical to the transliteration).
 __device__ __forceinline__ void interact_exact(float gm, float rx, float ry, float rz, float d2,
@@ -111,7 +113,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // finite); the self potential -m_i / eps is taken back once per leaf.  The
 // kernel picks the instance once per block, so the node loop carries no
 // test of it.
-template <bool EXACT, bool COUNT, bool FASTLEAF, bool QCHECK>
+template <bool EXACT, bool COUNT, bool FASTLEAF, bool GENERIC>
 __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
                                               const float* __restrict__ nsize,
                                               const float4* __restrict__ posm, float4 p,
@@ -127,17 +129,28 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
     // holds at most GS_NODE32_MAX = 2^27 records): the load takes the offset
     // register directly, with no 64-bit address arithmetic on the scalar
     // unit, which the walk keeps nearly as busy as the vector unit
-    const NodeRec rec = *(const NodeRec*)((const char*)nodes + ((uint32_t)nd << 5));
-    const float4 c = rec.cm;
-    const int4 inf = rec.info;
+    // (written as one instruction that waits for its data in the same asm
+    // block: left to itself the compiler may split it into narrower loads
+    // and re-load words inside the branches)
+    u32x8 w;
+    asm("s_load_dwordx8 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)"
+        : "=s"(w)
+        : "s"(nodes), "s"((uint32_t)nd << 5));
+    const float4 c = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
+                                 __uint_as_float(w[2]), __uint_as_float(w[3]));
+    const int4 inf = make_int4((int)w[4], (int)w[5], (int)w[6], (int)w[7]);
     const uint32_t meta = (uint32_t)inf.z;
     const uint32_t depth = gs_meta_depth(meta);
     const int32_t skip = inf.w;
     // a massless node is skipped by every lane (nbody_bh.wgsl:86): its depth
-    // test is made to fail on the scalar unit ((mass bits << 1) == 0)
-    uint32_t mbits;
-    asm("s_lshl_b32 %0, %1, 1" : "=s"(mbits) : "s"(__float_as_uint(c.w)));
-    const int32_t dtest = mbits != 0u ? (int32_t)depth : 64;
+    // test is made to fail on the scalar unit ((mass bits << 1) == 0).  Only
+    // the GENERIC variant tests it: a tree without massless nodes has none.
+    int32_t dtest = (int32_t)depth;
+    if (GENERIC) {
+      uint32_t mbits;
+      asm("s_lshl_b32 %0, %1, 1" : "=s"(mbits) : "s"(__float_as_uint(c.w)));
+      dtest = mbits != 0u ? (int32_t)depth : 64;
+    }
     // the lanes that opened every ancestor, as a wave mask (one compare)
     const uint64_t act = __builtin_amdgcn_ballot_w64(dtest <= alim);
     const float s2 = __int_as_float(inf.x);  // the record holds size * size
@@ -149,10 +162,10 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
     // first test is one compare per lane (false for a NaN d2, as in the
     // reference); the second depends on the node alone and is an integer
     // compare of non-negative float bits on the scalar unit, made only when
-    // some node of the tree is that small (QCHECK).  Every other active lane
+    // some node of the tree is that small (GENERIC).  Every other active lane
     // opens the node.
     uint64_t mono_raw = __builtin_amdgcn_ballot_w64(s2 < theta2 * d2);
-    if (QCHECK && (uint32_t)inf.x < qbits) mono_raw = ~0ull;
+    if (GENERIC && (uint32_t)inf.x < qbits) mono_raw = ~0ull;
     const uint64_t mono_mask = act & mono_raw;
     const uint64_t need_mask = act & ~mono_raw;
     if (__builtin_amdgcn_inverse_ballot_w64(mono_mask)) {
@@ -236,8 +249,8 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     float* __restrict__ energy_all, float4* __restrict__ acc_all, float* __restrict__ dens_all,
     uint32_t* __restrict__ inter_all, const NodeRec* __restrict__ pool,
     const float* __restrict__ nsize_all, const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot,
-    const DevParams* __restrict__ prms, const DevScalars* __restrict__ scs, DevBatch* batch,
-    double* __restrict__ red) {
+    const DevParams* __restrict__ prms, const DevScalars* __restrict__ scs,
+    const uint32_t* __restrict__ massless, DevBatch* batch, double* __restrict__ red) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
 
@@ -275,8 +288,10 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   const float* __restrict__ nsize = nsize_all + sl.noff;
 
   // the eps^2/4 test can only accept a node if some node of the tree has
-  // size^2 below eps^2/4 (never at depth <= 10 for the usual eps)
-  const bool qcheck = scs[slot].min_s2 < qeps2;
+  // size^2 below eps^2/4 (never at depth <= 10 for the usual eps), and the
+  // massless-node test only matters if the tree holds one: the walk runs
+  // without both tests unless the slot needs either
+  const bool generic = scs[slot].min_s2 < qeps2 || massless[slot] != 0u;
 
   float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
   uint32_t cnt = 0;
@@ -284,10 +299,10 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   walk_particle<EXACT, COUNT, FL, QC>(nodes, nsize, posm, p, i, valid, G, eps2, theta2,      \
                                       qbits, ax, ay, az, pot, dens, cnt)
   if (!EXACT && !COUNT && eps2 > 0.0f) {
-    if (qcheck) GS_WALK(true, true);
+    if (generic) GS_WALK(true, true);
     else GS_WALK(true, false);
   } else {
-    if (qcheck) GS_WALK(false, true);
+    if (generic) GS_WALK(false, true);
     else GS_WALK(false, false);
   }
 #undef GS_WALK
@@ -400,8 +415,8 @@ hipError_t launch(gs_sim* s) {
   const int c = s->cur;
   walk_kernel<EXACT, STEP, COUNT><<<s->walk_blocks, s->walk_wg, 0, s->stream>>>(
       s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
-      s->inter, s->nodes, s->nsize, s->slots, s->wblock_slot, s->prm, s->sc, s->batch,
-      s->red_part);
+      s->inter, s->nodes, s->nsize, s->slots, s->wblock_slot, s->prm, s->sc, s->massless,
+      s->batch, s->red_part);
   return hipGetLastError();
 }
 

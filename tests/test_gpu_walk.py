@@ -196,3 +196,54 @@ def test_hernquist_halo_closed_form():
     r = np.array([3.0, 0.0, 4.0])
     a = -5.0 * r / (5.0 * (5.0 + 2.0) ** 2)
     np.testing.assert_allclose(p[0, 4:7], a * 1e-3, rtol=1e-6)
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_massless_nodes_are_skipped(exact):
+    """Zero-mass particles make massless leaves and subtrees, which every lane
+    skips (nbody_bh.wgsl:86).  The refresh flags such a tree and the walk then
+    runs its variant with the massless-node test: forces against the oracle
+    after the on-device sort and after the host-tree upload (no refresh), then
+    again once the masses are back (the refresh clears the flag)."""
+    banner(f"massless nodes exact={exact}")
+    data = ic.make("cpu_disk_16k").copy()
+    data[data[:, 0] > 0.8, 3] = 0.0
+    ref = oracle_tree(data, bucket=16)
+    masses = ref["nodes"].reshape(-1, 16)[:, 3].view(np.float32)
+    assert (masses == 0).sum() > 10  # whole leaves and subtrees are massless
+    _, _, op = _params(data.shape[0], theta=0.7, epsilon=0.02)
+    _, _, racc, rdens, _ = oracle.walk(ref["sorted"], ref["nodes"], op)
+
+    def check(acc, dens):
+        if exact:
+            assert np.array_equal(acc.view(np.uint32), racc.view(np.uint32))
+        else:
+            assert rel_err(acc[:, :3], racc[:, :3]).max() <= FAST_TOL
+        assert np.array_equal(dens.view(np.uint32), rdens.view(np.uint32))
+
+    acc, dens, _ = _gpu_force(data, 16, exact, theta=0.7, epsilon=0.02)
+    check(acc, dens)
+    base, _, _ = _params(data.shape[0], theta=0.7, epsilon=0.02)
+    sim = galaxy_hip.NBodyGPU(data, bucket=16, exact_force=exact)
+    try:
+        sim.update_params(**base)
+        sim.apply_permutation(ref["perm"])
+        sim.upload_tree(ref["nodes"], ref["num_nodes"], ref["level_ranges"])
+        sim.force()
+        check(*sim.accel())
+        # masses back: the same handle's refreshed tree has no massless node
+        full = ic.make("cpu_disk_16k")
+        ref2 = oracle_tree(full, bucket=16)
+        _, _, racc2, rdens2, _ = oracle.walk(ref2["sorted"], ref2["nodes"], op)
+        sim.upload(full)
+        sim.sort()
+        sim.refresh()
+        sim.force()
+        acc2, dens2 = sim.accel()
+    finally:
+        sim.destroy()
+    if exact:
+        assert np.array_equal(acc2.view(np.uint32), racc2.view(np.uint32))
+    else:
+        assert rel_err(acc2[:, :3], racc2[:, :3]).max() <= FAST_TOL
+    assert np.array_equal(dens2.view(np.uint32), rdens2.view(np.uint32))
