@@ -372,7 +372,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true",
                     help="batched run: time the pooled steps only (profiling)")
     ap.add_argument("--plain-steps", type=int, default=5)
-    ap.add_argument("--serve-jobs", type=int, default=120)
+    ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--serve-resident", type=int, default=8 << 20,
                     help="resident particle limit of the server")
@@ -396,7 +396,11 @@ def main():
         print(json.dumps(bench_batch(args)))
         return
     if args.only == "serve":
-        print(json.dumps(bench_serve(args, {})))
+        tuned = {}
+        if args.tuned:
+            wg, bucket, radix = (int(x) for x in args.tuned.split(","))
+            tuned = {"walk_wg": wg, "bucket": bucket, "radix_bits": radix}
+        print(json.dumps(bench_serve(args, tuned)))
         return
     traffic = None
     if not (args.no_pmc or args.only == "headline"):

@@ -610,7 +610,8 @@ int gs_pool_info(gs_sim* s, uint32_t* nslots, uint64_t* device_bytes) {
     if (device_bytes) {
       const uint64_t P = s->ptot;
       *device_bytes = 2 * P * 32 + 4 * P * 4 + P * 4 + P * (4 + 16 + 4 + 4) +
-                      (uint64_t)s->ntot * sizeof(NodeRec) + 2ull * s->build_words * 8 +
+                      (uint64_t)s->ntot * (sizeof(NodeRec) + sizeof(float)) +
+                      2ull * s->build_words * 8 +
                       (uint64_t)GS_HIST_WORDS * s->sort_tiles * 4 +
                       (uint64_t)s->walk_blocks * GS_NRED * 8;
     }
@@ -1133,7 +1134,7 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     float min_s2 = INFINITY;
     for (int d = 0; d <= GS_MAX_LEVELS; d++) {
       sc.level_size[d] = (float)sz;
-      min_s2 = fminf(min_s2, host_s2(sc.level_size[d]));
+      if (d <= GS_MAX_LEVEL) min_s2 = fminf(min_s2, host_s2(sc.level_size[d]));
       sz *= 0.5;
     }
     sc.min_s2 = min_s2;

@@ -175,7 +175,8 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
     const float f = __uint_as_float(sd::to_f32(sz));
     sc->level_size[dpt] = f;
-    min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));  // fminf drops a NaN
+    // nodes exist at depths 0..GS_MAX_LEVEL only; fminf drops a NaN
+    if (dpt <= GS_MAX_LEVEL) min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));
     if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
   }
   sc->min_s2 = min_s2;

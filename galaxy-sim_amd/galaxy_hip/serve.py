@@ -30,9 +30,10 @@ from . import GalaxyPool
 
 # device bytes per particle of resident state: 2 x (posm + velp) ping-pong,
 # 2 x (key + index) sort ping-pong, raw code, energy, acceleration, density,
-# interaction count; plus a node pool of 2n + 4096 32-byte records
+# interaction count; plus a node pool of 2n + 4096 32-byte records and their
+# sizes
 PARTICLE_STATE_BYTES = 2 * 32 + 2 * 8 + 4 + 4 + 16 + 4 + 4
-NODE_BYTES = 32
+NODE_BYTES = 32 + 4
 
 
 def state_bytes(n):

@@ -44,7 +44,8 @@ def _gpu_force(data, bucket, exact, **kw):
     return acc, dens, sorted_
 
 
-@pytest.mark.parametrize("theta,eps", [(0.7, 0.02), (0.5, 0.005), (0.0, 0.02)])
+# eps 0.5: nodes below eps/2 in size are accepted by the s^2 < eps^2/4 test
+@pytest.mark.parametrize("theta,eps", [(0.7, 0.02), (0.5, 0.005), (0.0, 0.02), (0.7, 0.5)])
 def test_exact_force_bit_identical(theta, eps):
     banner(f"exact force theta={theta} eps={eps}")
     data = ic.make("cpu_disk_16k")
