@@ -253,12 +253,12 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
               nsize[c] = size_c;
               nodes[c].info = make_int4(s2_c, (int)b0,
                                          (int)gs_meta_open((uint32_t)d + 1, b1 - b0),
-                                         c > fc ? (int32_t)c - 1 : skip);
+                                         c > fc ? gs_link(c - 1) : skip);
             }
             c++;
           }
         }
-        nodes[idx].info = make_int4(s2_d, (int)(fc + nchild - 1),  // last child
+        nodes[idx].info = make_int4(s2_d, gs_link(fc + nchild - 1),  // last child
                                      (int)gs_meta_internal((uint32_t)d, mask, nchild), skip);
       }
     }
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void node64_kernel(const NodeRec* __restrict__
     w[6] = leaf ? (uint32_t)inf.y : 0u;
     w[7] = leaf ? gs_meta_count(meta) : 0u;
     const uint32_t mask = leaf ? 0u : gs_meta_mask(meta);
-    uint32_t c0 = (uint32_t)inf.y + 1u - gs_meta_nc(meta);  // internal: y = last child
+    uint32_t c0 = gs_node(inf.y) + 1u - gs_meta_nc(meta);  // internal: y = last child
     for (int k = 0; k < 8; k++) {
       if (mask & (1u << k)) {
         w[8 + k] = c0++;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nod
     const uint4 a = r[0], b = r[1], c = r[2], d = r[3];
     nodes[i].cm = make_float4(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z),
                               __uint_as_float(a.w));
-    const int32_t skip = links[i].x;
+    const int32_t skip = links[i].x < 0 ? -1 : gs_link((uint32_t)links[i].x);
     const uint32_t depth = (uint32_t)links[i].y;
     const float size = __uint_as_float(b.x);
     const int s2 = gs_s2_bits(size);
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nod
       }
     }
     // internal: y = last child (a childless node continues at its skip)
-    nodes[i].info = make_int4(s2, nc ? (int)(fc + nc - 1) : skip,
+    nodes[i].info = make_int4(s2, nc ? gs_link(fc + nc - 1) : skip,
                               (int)gs_meta_internal(depth, mask, nc), skip);
   }
 }

@@ -18,6 +18,9 @@
 //   node[C]     32-byte record (one s_load_dwordx8 in the walk):
 //               float4 cm   (comx, comy, comz, mass)
 //               int4   info (size*size bits, first | lastChild, meta, skip)
+//               (lastChild and skip are byte offsets of 32-byte records,
+//               i.e. 32 x the node index, so the walk loads a record without
+//               an index shift; -1 ends the walk)
 //               meta: bit 31 leaf, bits 27..30 depth, then
 //                     leaf: bits 0..26 particle count
 //                     internal: bits 0..3 child count, bits 8..15 octant mask
@@ -144,10 +147,13 @@ __host__ __device__ __forceinline__ int gs_s2_bits(float size) {
 // node meta word
 #define GS_META_LEAF 0x80000000u
 #define GS_MAX_N ((1u << 27) - 1)  // a leaf count must fit the 27-bit field
-// a slot's node pool holds at most this many records (2^27 x 32 bytes =
-// 4 GiB), so the walk addresses them by 32-bit byte offsets; slots above 2^26
-// particles get this cap instead of 2N + 4096 records
-#define GS_NODE32_MAX (1u << 27)
+// a slot's node pool holds at most this many records (2^26 x 32 bytes =
+// 2 GiB), so node links are non-negative 32-bit byte offsets; slots above
+// 2^25 - 2048 particles get this cap instead of 2N + 4096 records
+#define GS_NODE32_MAX (1u << 26)
+// node index <-> node link (byte offset of the 32-byte record)
+__host__ __device__ __forceinline__ int32_t gs_link(uint32_t node) { return (int32_t)(node << 5); }
+__host__ __device__ __forceinline__ uint32_t gs_node(int32_t link) { return (uint32_t)link >> 5; }
 __host__ __device__ __forceinline__ uint32_t gs_meta_leaf(uint32_t depth, uint32_t count) {
   return GS_META_LEAF | (depth << 27) | count;
 }

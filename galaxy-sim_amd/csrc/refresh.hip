@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
     if (gs_meta_is_leaf((uint32_t)inf.z)) continue;
     double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     const uint32_t nc = gs_meta_nc((uint32_t)inf.z);
-    const uint32_t fc = (uint32_t)inf.y + 1u - nc;  // y = last child
+    const uint32_t fc = gs_node(inf.y) + 1u - nc;  // y = last child
     for (uint32_t c = fc; c < fc + nc; c++) {
       const float4 q = nodes[c].cm;
       const double qm = (double)q.w;

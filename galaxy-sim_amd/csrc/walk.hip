@@ -123,19 +123,23 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
                                               float& dens, uint32_t& cnt) {
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
   const float selfpot = FASTLEAF ? p.w * __builtin_amdgcn_rsqf(eps2) : 0.f;
-  int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;
+  int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;  // node link (record byte offset)
+  // the kernel's own scalar loads (parameters, slot) complete here, so the
+  // loop below carries no wait for them
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
   while (nd >= 0) {
-    // one 32-byte scalar load at a 32-bit byte offset (a slot's node pool
-    // holds at most GS_NODE32_MAX = 2^27 records): the load takes the offset
-    // register directly, with no 64-bit address arithmetic on the scalar
-    // unit, which the walk keeps nearly as busy as the vector unit
+    // one 32-byte scalar load at the node link, a 32-bit byte offset (a
+    // slot's node pool holds at most GS_NODE32_MAX = 2^26 records): the load
+    // takes the link register directly, with no shift or 64-bit address
+    // arithmetic on the scalar unit, which the walk keeps nearly as busy as
+    // the vector unit
     // (written as one instruction that waits for its data in the same asm
     // block: left to itself the compiler may split it into narrower loads
     // and re-load words inside the branches)
     u32x8 w;
     asm("s_load_dwordx8 %0, %1, %2\n\ts_waitcnt lgkmcnt(0)"
         : "=s"(w)
-        : "s"(nodes), "s"((uint32_t)nd << 5));
+        : "s"(nodes), "s"(nd));
     const float4 c = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
                                  __uint_as_float(w[2]), __uint_as_float(w[3]));
     const int4 inf = make_int4((int)w[4], (int)w[5], (int)w[6], (int)w[7]);
@@ -184,7 +188,7 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       // lane that opens its own leaf takes back the self potential that the
       // member loop below adds.
       if (__builtin_amdgcn_inverse_ballot_w64(act) && i - first < count) {
-        dens = (float)count / (s2 * nsize[nd]);
+        dens = (float)count / (s2 * nsize[gs_node(nd)]);
         if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
       }
       if (__builtin_amdgcn_inverse_ballot_w64(need_mask)) {  // lanes opening the leaf
@@ -224,20 +228,21 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
         }
       }
     }
-    // depth limit: alim = min(alim, depth) + desc.  On a descent the opening
-    // lanes (alim >= depth) become active at the children's depth and the
-    // others only above this node; after any other visit the next node is no
-    // deeper than this one, where the update changes no lane's activity, so
-    // it runs unconditionally (the descent mask is the add's carry-in)
+    // next node: an opened internal node continues at its last child
+    // (visited first), leaves and unopened nodes after the subtree.  Only a
+    // descent updates the depth limit, alim = min(alim, depth) + desc: the
+    // opening lanes become active at the children's depth and the others
+    // only above this node; after any other visit the next node is no deeper
+    // than this one, where the update would change no lane's activity.
     uint64_t carry;
-    asm("v_min_i32_e32 %0, %2, %0\n\tv_addc_co_u32_e64 %0, %1, 0, %0, %3"
-        : "+v"(alim), "=&s"(carry)
-        : "s"(depth), "s"(desc_mask));
-    // an opened internal node continues at its last child (visited first);
-    // leaves and unopened nodes after the subtree
-    asm("s_cmp_eq_u64 %1, 0\n\ts_cselect_b32 %0, %2, %3"
-        : "=s"(nd)
-        : "s"(desc_mask), "s"(skip), "s"(first)
+    asm("s_cmp_eq_u64 %[dm], 0\n\t"
+        "s_cselect_b32 %[nd], %[skip], %[first]\n\t"
+        "s_cbranch_scc1 1f\n\t"
+        "v_min_i32_e32 %[alim], %[depth], %[alim]\n\t"
+        "v_addc_co_u32_e64 %[alim], %[carry], 0, %[alim], %[dm]\n"
+        "1:"
+        : [nd] "=&s"(nd), [alim] "+v"(alim), [carry] "=&s"(carry)
+        : [dm] "s"(desc_mask), [skip] "s"(skip), [first] "s"(first), [depth] "s"(depth)
         : "scc");
   }
 }

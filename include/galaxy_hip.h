@@ -61,9 +61,9 @@ typedef struct gs_config {
   uint32_t bucket;    /* leaf capacity BUCKET (mortonTree.js:25); 0 -> 16 */
   uint64_t max_nodes; /* leaf-rule pool size maxNodes (mortonTree.js:88,127);
                          0 -> 2n + 4096.  The device pool holds
-                         min(max(2n + 4096, max_nodes), 2^27) nodes (the
+                         min(max(2n + 4096, max_nodes), 2^26) nodes (the
                          walk addresses them by 32-bit byte offsets);
-                         max_nodes at most 2^27. */
+                         max_nodes at most 2^26. */
   uint32_t flags;     /* GS_FLAG_* */
   int32_t device;     /* HIP device ordinal, -1 = current device */
   uint32_t walk_wg;   /* walk workgroup size, a power of two in [64, 1024];
