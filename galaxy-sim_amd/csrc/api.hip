@@ -71,10 +71,22 @@ struct Stage {
   gs_sim* s;
   int id;
   hipEvent_t a = nullptr, b = nullptr;
+  // events come from the handle's spare list (created once, reused every
+  // step), so timing adds no event creation to a step
+  static hipEvent_t take(gs_sim* s) {
+    if (!s->spare_events.empty()) {
+      hipEvent_t e = s->spare_events.back();
+      s->spare_events.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
   Stage(gs_sim* sim, int stage) : s(sim), id(stage) {
-    if (!(s->flags & GS_FLAG_TIMING)) return;
-    check(hipEventCreate(&a), "hipEventCreate");
-    check(hipEventCreate(&b), "hipEventCreate");
+    if (!(s->flags & GS_FLAG_TIMING) || s->capturing) return;
+    a = take(s);
+    b = take(s);
     check(hipEventRecord(a, s->stream), "hipEventRecord");
   }
   ~Stage() {
@@ -82,8 +94,8 @@ struct Stage {
     if (hipEventRecord(b, s->stream) == hipSuccess) {
       s->timers.push_back(StageTimer{id, a, b});
     } else {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
+      s->spare_events.push_back(a);
+      s->spare_events.push_back(b);
     }
   }
 };
@@ -97,8 +109,8 @@ void collect_timers(gs_sim* s) {
       s->t_ms[t.stage] += ms;
       s->t_cnt[t.stage] += 1;
     }
-    (void)hipEventDestroy(t.a);
-    (void)hipEventDestroy(t.b);
+    s->spare_events.push_back(t.a);
+    s->spare_events.push_back(t.b);
   }
   s->timers.clear();
 }
@@ -118,13 +130,14 @@ void check_status(gs_sim* s) {
 }
 
 void destroy_graphs(gs_sim* s) {
-  for (int p = 0; p < 2; p++)
-    for (int q = 0; q < 2; q++)
-      for (int r = 0; r < 2; r++)
-        if (s->graph[p][q][r]) {
-          (void)hipGraphExecDestroy(s->graph[p][q][r]);
-          s->graph[p][q][r] = nullptr;
-        }
+  for (auto& a : s->graph)
+    for (auto& b : a)
+      for (auto& c : b)
+        for (hipGraphExec_t& g : c)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
 }
 
 void free_all(gs_sim* s) {
@@ -166,6 +179,8 @@ void free_all(gs_sim* s) {
     (void)hipEventDestroy(t.b);
   }
   s->timers.clear();
+  for (hipEvent_t e : s->spare_events) (void)hipEventDestroy(e);
+  s->spare_events.clear();
   if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -230,8 +245,10 @@ void launch_sort(gs_sim* s, bool keep_raw) {
 }
 
 void launch_refresh(gs_sim* s) {
-  Stage t(s, GS_T_REFRESH);
-  check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
+  {
+    Stage t(s, GS_T_REFRESH);
+    check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
+  }
 }
 
 void launch_walk_step(gs_sim* s) {
@@ -256,22 +273,27 @@ void mark_stepped(gs_sim* s) {
   }
 }
 
-bool use_graph(const gs_sim* s) { return !(s->flags & (GS_FLAG_NO_GRAPH | GS_FLAG_TIMING)); }
+bool use_graph(const gs_sim* s) { return !(s->flags & GS_FLAG_NO_GRAPH); }
 
 // One step (optional rebuild + refresh + walk) as a graph for the current
 // ping-pong parity; the launch sequence depends only on static sizes, so one
 // capture serves every later step and every change of slot occupancy.
-hipGraphExec_t step_graph(gs_sim* s, bool sort) {
-  hipGraphExec_t& slot = s->graph[s->cur][s->stel_cur][sort ? 1 : 0];
+// part 0: the whole step; 1: rebuild (if sort) + refresh; 2: the walk
+hipGraphExec_t step_graph(gs_sim* s, bool sort, int part) {
+  hipGraphExec_t& slot = s->graph[s->cur][s->stel_cur][sort ? 1 : 0][part];
   if (slot) return slot;
   const int cur0 = s->cur, stel0 = s->stel_cur;
   hipGraph_t g = nullptr;
   check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  s->capturing = true;  // no stage events inside a capture
   try {
-    if (sort) launch_sort(s, false);
-    launch_refresh(s);
-    launch_walk_step(s);
+    if (part != 2) {
+      if (sort) launch_sort(s, false);
+      launch_refresh(s);
+    }
+    if (part != 1) launch_walk_step(s);
   } catch (...) {
+    s->capturing = false;
     hipGraph_t dead = nullptr;
     (void)hipStreamEndCapture(s->stream, &dead);
     if (dead) (void)hipGraphDestroy(dead);
@@ -279,6 +301,7 @@ hipGraphExec_t step_graph(gs_sim* s, bool sort) {
     s->stel_cur = stel0;
     throw;
   }
+  s->capturing = false;
   check(hipStreamEndCapture(s->stream, &g), "end capture");
   s->cur = cur0;
   s->stel_cur = stel0;
@@ -291,8 +314,22 @@ hipGraphExec_t step_graph(gs_sim* s, bool sort) {
 }
 
 void run_step(gs_sim* s, bool sort) {
-  if (use_graph(s)) {
-    hipGraphExec_t ex = step_graph(s, sort);
+  if (use_graph(s) && (s->flags & GS_FLAG_TIMING)) {
+    // two replays with events between them: the walk and the whole step are
+    // timed on the stream that runs them
+    Stage t(s, GS_T_STEP);
+    check(hipGraphLaunch(step_graph(s, sort, 1), s->stream), "graph launch");
+    if (sort) {  // the sort's gather flipped the buffers
+      s->cur ^= 1;
+      if (s->stel[0]) s->stel_cur ^= 1;
+    }
+    {
+      Stage w(s, GS_T_WALK);
+      check(hipGraphLaunch(step_graph(s, false, 2), s->stream), "graph launch");
+    }
+    s->cur ^= 1;
+  } else if (use_graph(s)) {
+    hipGraphExec_t ex = step_graph(s, sort, 0);
     check(hipGraphLaunch(ex, s->stream), "graph launch");
     s->cur ^= sort ? 0 : 1;  // the sort's gather flips once, the walk once
     if (sort && s->stel[0]) s->stel_cur ^= 1;
@@ -824,7 +861,10 @@ int gs_step(gs_sim* s, uint32_t nsteps, uint32_t sort_every) {
 }
 
 int gs_sync(gs_sim* s) {
-  return guarded(s, [&] { check_status(s); });
+  return guarded(s, [&] {
+    check_status(s);
+    collect_timers(s);  // the stream is idle: fold the finished stage events in
+  });
 }
 
 int gs_download_particles(gs_sim* s, float* dst, uint64_t n) {

@@ -280,10 +280,14 @@ struct gs_sim {
   uint64_t step_count = 0;
 
   std::vector<StageTimer> timers;
+  std::vector<hipEvent_t> spare_events;  // timing events reused across steps
+  bool capturing = false;                 // a step graph is being captured
   double t_ms[GS_NT] = {0};
   uint64_t t_cnt[GS_NT] = {0};
 
-  hipGraphExec_t graph[2][2][2] = {};  // [particle parity][stellar parity][sort]
+  // step graphs [particle parity][stellar parity][sort][part]: part 0 the
+  // whole step, 1 rebuild + refresh, 2 walk (timing handles replay 1 then 2)
+  hipGraphExec_t graph[2][2][2][3] = {};
   bool graph_dirty = true;
 
   std::string err;

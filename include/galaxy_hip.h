@@ -50,7 +50,10 @@ extern "C" {
 #define GS_FLAG_EXACT_FORCE 0x1u /* IEEE sqrt and '/' in every interaction:   \
                                     forces bit-identical to the transliterated \
                                     WGSL; default is the rsqrt fast path */
-#define GS_FLAG_TIMING 0x2u      /* time every stage with HIP events */
+#define GS_FLAG_TIMING 0x2u      /* time stages with HIP events; gs_step  \
+                                    then replays two graphs per step        \
+                                    (rebuild + refresh, walk) and times the \
+                                    walk and the whole step */
 #define GS_FLAG_NO_GRAPH 0x4u    /* gs_step launches kernels one by one
                                     instead of replaying a captured graph */
 
