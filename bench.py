@@ -274,15 +274,19 @@ def probe_walk(args):
 
 
 def _pmc_pass(counter, args, workdir):
-    """One rocprofv3 counter pass over probe_walk; per-kernel mean counter
-    value per dispatch (KiB for FETCH_SIZE / WRITE_SIZE)."""
+    """One rocprofv3 counter pass over probe_walk (counter: one name, or
+    several separated by spaces); per counter, the mean value per walk
+    dispatch and the largest copy dispatch (KiB for FETCH_SIZE / WRITE_SIZE).
+    A single counter returns {"walk", "copy"}; several return one such dict
+    per counter."""
+    names = counter.split()
     import shutil
     import signal
     import sqlite3
     import subprocess
     exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
-    outdir = os.path.join(workdir, counter.lower())
-    cmd = [exe, "--pmc", counter, "--kernel-include-regex", "walk_kernel|copyBuffer",
+    outdir = os.path.join(workdir, "_".join(names).lower())
+    cmd = [exe, "--pmc", *names, "--kernel-include-regex", "walk_kernel|copyBuffer",
            "-d", outdir, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__),
            "--probe-walk", "--n", str(args.n)]
     env = dict(os.environ, TMPDIR="/tmp")
@@ -297,21 +301,24 @@ def _pmc_pass(counter, args, workdir):
     dbs = [os.path.join(r, f) for r, _, fs in os.walk(outdir) for f in fs if f.endswith(".db")]
     if rc != 0 and not dbs:
         raise RuntimeError(f"{counter} pass exited {rc}")
-    vals = {}
-    for db in dbs:
-        con = sqlite3.connect(db)
-        for name, value in con.execute(
-                "select kernel_name, value from counters_collection where counter_name = ?",
-                (counter,)):
-            key = "walk" if "walk_kernel" in name else "copy" if "copyBuffer" in name else None
-            if key:
-                vals.setdefault(key, []).append(float(value))
-        con.close()
-    if "walk" not in vals or "copy" not in vals:
-        raise RuntimeError(f"{counter} pass recorded no walk/copy dispatches")
-    # the calibration copy is the largest copy dispatch (host uploads are
-    # smaller blit copies of the same kernel)
-    return {"walk": float(np.mean(vals["walk"])), "copy": float(max(vals["copy"]))}
+    out = {}
+    for c in names:
+        vals = {}
+        for db in dbs:
+            con = sqlite3.connect(db)
+            for name, value in con.execute(
+                    "select kernel_name, value from counters_collection where counter_name = ?",
+                    (c,)):
+                key = "walk" if "walk_kernel" in name else "copy" if "copyBuffer" in name else None
+                if key:
+                    vals.setdefault(key, []).append(float(value))
+            con.close()
+        if "walk" not in vals or "copy" not in vals:
+            raise RuntimeError(f"{c} pass recorded no walk/copy dispatches")
+        # the calibration copy is the largest copy dispatch (host uploads are
+        # smaller blit copies of the same kernel)
+        out[c] = {"walk": float(np.mean(vals["walk"])), "copy": float(max(vals["copy"]))}
+    return out[names[0]] if len(names) == 1 else out
 
 
 def pmc_traffic(args):
@@ -327,6 +334,10 @@ def pmc_traffic(args):
         scale = known_kib / v["copy"]
         res[counter] = {"walk_kib": v["walk"], "copy_kib": v["copy"], "scale": scale,
                         "walk_bytes": v["walk"] * scale * 1024.0}
+    # L2 (TCC) hit rate of the walk: hits / (hits + misses) over all channels
+    v = _pmc_pass("TCC_HIT_sum TCC_MISS_sum", args, work)
+    hit, miss = v["TCC_HIT_sum"]["walk"], v["TCC_MISS_sum"]["walk"]
+    res["L2"] = {"hits": hit, "misses": miss, "hit_rate": hit / max(hit + miss, 1.0)}
     return res
 
 
@@ -362,7 +373,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--no-pmc", action="store_true",
-                    help="skip the two rocprofv3 counter passes that measure walk HBM traffic")
+                    help="skip the rocprofv3 counter passes (walk HBM traffic, L2 hit rate)")
     ap.add_argument("--pmc-timeout", type=int, default=120)
     ap.add_argument("--probe-walk", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-galaxy batched run")
@@ -407,7 +418,7 @@ def main():
     traffic = None
     if not (args.no_pmc or args.only == "headline"):
         try:
-            log("walk HBM traffic: two rocprofv3 counter passes")
+            log("walk HBM traffic and L2 hit rate: three rocprofv3 counter passes")
             traffic = pmc_traffic(args)
         except Exception as e:
             log(f"counter passes failed ({type(e).__name__}: {e}); traffic = null")
@@ -424,6 +435,10 @@ def main():
             "hbm_gbs": (fetch + write) / (out["roofline"]["walk_avg_ms"] * 1e-3) / 1e9,
             "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per walk dispatch, scaled by a "
                       f"{COPY_PROBE_MIB} MiB device copy"}
+        out["roofline"]["l2_hit_rate"] = traffic["L2"]["hit_rate"]
+        out["roofline"]["l2_detail"] = {
+            "hits": traffic["L2"]["hits"], "misses": traffic["L2"]["misses"],
+            "source": "rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum per walk dispatch"}
     tuned = {}
     if not (args.no_batch or args.only == "headline"):
         try:
