@@ -52,9 +52,9 @@ def _alone(data, bucket, params, exact, nsteps, sort_every=1):
         sim.destroy()
 
 
-def _make_pool(exact, graph=True, jobs=JOBS):
+def _make_pool(exact, graph=True, jobs=JOBS, timing=False):
     pool = galaxy_hip.GalaxyPool([j[0] for j in jobs], buckets=[j[3] for j in jobs],
-                                 exact_force=exact, graph=graph)
+                                 exact_force=exact, graph=graph, timing=timing)
     datas = []
     for k, (cap, n, preset, bucket, extra) in enumerate(jobs):
         data = _galaxy(preset, n, k)
@@ -138,6 +138,43 @@ def test_pool_graph_equals_direct_launches():
             pool.destroy()
     for a, b in zip(*out):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_timed_replays_equal_plain_replays():
+    """A timing handle replays each step as two graphs (rebuild + refresh,
+    walk) with events between them: same particles as one graph per step and
+    as direct launches, for a pool and for a single galaxy; the walk and the
+    whole step are timed once per step, the other stages by direct launches."""
+    banner("timed replays")
+    out = []
+    for graph, timing in ((True, False), (True, True), (False, True)):
+        pool, datas = _make_pool(False, graph=graph, timing=timing)
+        try:
+            for _ in range(3):  # stage times are folded in at each sync
+                pool.step(1, sort_every=2)
+                pool.sync()
+            out.append([pool.particles(k) for k in range(len(JOBS))])
+            st = pool.stage_times()
+        finally:
+            pool.destroy()
+        if timing:
+            assert st["walk"][1] == 3 and st["step"][1] == 3 and st["walk"][0] > 0
+            assert (st["sort"][1] > 0) == (not graph)
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    data = _galaxy("disk", 20000, 7)
+    res = []
+    for graph, timing in ((True, False), (True, True), (False, True)):
+        sim = galaxy_hip.NBodyGPU(data, bucket=16, graph=graph, timing=timing)
+        try:
+            sim.update_params(**_params(dict(theta=0.6, epsilon=0.01)))
+            sim.step(3, sort_every=2)
+            res.append(sim.particles())
+        finally:
+            sim.destroy()
+    for r in res[1:]:
+        assert np.array_equal(res[0].view(np.uint32), r.view(np.uint32))
 
 
 def test_pool_slot_reuse():
