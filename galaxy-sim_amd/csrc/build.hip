@@ -25,6 +25,12 @@
 
 namespace {
 
+// phase 1 gives each node of a tile 8 lanes (one per octant) and covers the
+// tile in rows of 32 nodes, and gs_slot_prefix needs a thread per slot: the
+// tile is exactly 256 nodes, one per thread
+static_assert(GS_BUILD_TILE == 256, "build phase 1 maps 8 lanes x 32 nodes per row");
+static_assert(GS_BUILD_TILE >= GS_MAX_SLOTS, "slot prefix needs a thread per slot");
+
 constexpr uint64_t kAgg = 1ull << 32;
 constexpr uint64_t kPre = 2ull << 32;
 
