@@ -79,6 +79,9 @@ __host__ __device__ constexpr int gs_sort_passes(int bits) { return (30 + bits -
 #define GS_SORT_BLOCK 256
 #define GS_SORT_ITEMS 16
 #define GS_SORT_TILE (GS_SORT_BLOCK * GS_SORT_ITEMS)
+// the sort's tile table is built from the encode chunks: a sort tile is one
+// encode chunk
+static_assert(GS_SORT_TILE == GS_CHUNK, "sort tiles and encode chunks must coincide");
 
 #define GS_BUILD_TILE 256
 
