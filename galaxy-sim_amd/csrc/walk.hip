@@ -96,12 +96,6 @@ __device__ __forceinline__ void pair_fast(float4 q, float4 p, float eps2, float&
   interact_fast(q.w, qx, qy, qz, dj2, ax, ay, az, pot);
 }
 
-__device__ __forceinline__ uint64_t rfl64(uint64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
