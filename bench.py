@@ -335,9 +335,12 @@ def pmc_traffic(args):
         res[counter] = {"walk_kib": v["walk"], "copy_kib": v["copy"], "scale": scale,
                         "walk_bytes": v["walk"] * scale * 1024.0}
     # L2 (TCC) hit rate of the walk: hits / (hits + misses) over all channels
-    v = _pmc_pass("TCC_HIT_sum TCC_MISS_sum", args, work)
-    hit, miss = v["TCC_HIT_sum"]["walk"], v["TCC_MISS_sum"]["walk"]
-    res["L2"] = {"hits": hit, "misses": miss, "hit_rate": hit / max(hit + miss, 1.0)}
+    try:
+        v = _pmc_pass("TCC_HIT_sum TCC_MISS_sum", args, work)
+        hit, miss = v["TCC_HIT_sum"]["walk"], v["TCC_MISS_sum"]["walk"]
+        res["L2"] = {"hits": hit, "misses": miss, "hit_rate": hit / max(hit + miss, 1.0)}
+    except Exception as e:  # the traffic figures stand without it
+        log(f"L2 counter pass failed ({type(e).__name__}: {e}); l2_hit_rate = null")
     return res
 
 
@@ -435,10 +438,11 @@ def main():
             "hbm_gbs": (fetch + write) / (out["roofline"]["walk_avg_ms"] * 1e-3) / 1e9,
             "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per walk dispatch, scaled by a "
                       f"{COPY_PROBE_MIB} MiB device copy"}
-        out["roofline"]["l2_hit_rate"] = traffic["L2"]["hit_rate"]
-        out["roofline"]["l2_detail"] = {
-            "hits": traffic["L2"]["hits"], "misses": traffic["L2"]["misses"],
-            "source": "rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum per walk dispatch"}
+        out["roofline"]["l2_hit_rate"] = traffic["L2"]["hit_rate"] if "L2" in traffic else None
+        if "L2" in traffic:
+            out["roofline"]["l2_detail"] = {
+                "hits": traffic["L2"]["hits"], "misses": traffic["L2"]["misses"],
+                "source": "rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum per walk dispatch"}
     tuned = {}
     if not (args.no_batch or args.only == "headline"):
         try:
