@@ -277,3 +277,33 @@ def test_exact_steps_at_full_size(name, nsteps):
     want, went = _oracle_steps(data, nsteps, c["bucket"], op)
     assert np.array_equal(parts.reshape(-1).view(np.uint32), want.view(np.uint32))
     assert np.array_equal(energy.view(np.uint32), went.view(np.uint32))
+
+
+def test_exact_rebuild_cadence():
+    """The reference's frame schedule (main.js:401-477): the topology is rebuilt
+    every k-th step and the steps in between refresh the stale tree's COMs and
+    walk it.  Six EXACT steps with a rebuild every third equal the oracle
+    replaying the same schedule bit for bit."""
+    banner("exact rebuild cadence, sort every 3 steps")
+    data = ic.make("cpu_disk_16k", n=20000)
+    n = data.shape[0]
+    base, halo, op = _params(n, theta=0.6, epsilon=0.01, gasDrag=0.25)
+    sim = galaxy_hip.NBodyGPU(data, bucket=16, exact_force=True)
+    try:
+        sim.update_params(haloCenters=halo, **base)
+        sim.step(6, sort_every=3)
+        parts = sim.particles()
+        energy = sim.energy()
+    finally:
+        sim.destroy()
+    cur, nodes = data.reshape(-1).copy(), None
+    went = None
+    for s in range(6):
+        if s % 3 == 0:
+            ref = oracle.sort_and_build(cur, n, bucket=16)
+            cur, nodes = ref["sorted"], ref["nodes"]
+        else:
+            nodes = oracle.refresh_f64(cur, nodes)
+        cur, went, _, _, _ = oracle.walk(cur, nodes, op)
+    assert np.array_equal(parts.reshape(-1).view(np.uint32), cur.view(np.uint32))
+    assert np.array_equal(energy.view(np.uint32), went.view(np.uint32))
