@@ -105,6 +105,23 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
       const uint32_t k = threadIdx.x & 7;
       uint32_t sfirst[R], slen[R], stgt[R];
       bool more = false;
+      // the R node records, then the first code of each node, are loaded
+      // together (rows past the level's end load the level's first node, and
+      // nodes without a valid range load code 0 of a non-empty slot), so the
+      // set-up costs two memory latencies rather than 2R
+      int4 infr[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const uint32_t local = tile * GS_BUILD_TILE + (uint32_t)r * 32 + (threadIdx.x >> 3);
+        infr[r] = nodes[start + (local < count ? local : 0u)].info;
+      }
+      uint32_t c0[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const uint32_t lo = (uint32_t)infr[r].y;
+        c0[r] = 0;
+        if (n > 0) c0[r] = codes[lo < n ? lo : 0u];  // n is uniform over the block
+      }
 #pragma unroll
       for (int r = 0; r < R; r++) {
         const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
@@ -114,33 +131,63 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
         stgt[r] = 0;
         if (local < count) {
           const uint32_t idx = start + local;
-          const int4 inf = nodes[idx].info;
+          const int4 inf = infr[r];
           const uint32_t lo = (uint32_t)inf.y, cnt = gs_meta_count((uint32_t)inf.z);
           const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
           sfirst[r] = lo;
           if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
             slen[r] = cnt;
-            stgt[r] = ((codes[lo] >> (shift + 3)) << (shift + 3)) | (k << shift);
+            stgt[r] = ((c0[r] >> (shift + 3)) << (shift + 3)) | (k << shift);
             more = true;
           }
           if (k == 0) s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
         }
       }
-      // lower_bound over [first, first + len) for every r at once
+      // lower_bound over [first, first + len) for every r at once, two
+      // bisection steps per round: the probe of the first step and both
+      // possible probes of the second are loaded together, with no branch
+      // between the loads, so a round costs one memory latency for all R
+      // searches and both steps.  Every probe lies in [first, first + len),
+      // inside the node's particle range; a finished search (len 0) probes
+      // index 0, which exists whenever a lane is still in the loop (its node
+      // is non-empty).
       while (more) {
+        uint32_t vm[R], vl[R], vr[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const uint32_t len = slen[r], half = len >> 1, rest = len - half - 1;
+          const uint32_t am = len > 0 ? sfirst[r] + half : 0u;
+          const uint32_t al = half > 0 ? sfirst[r] + (half >> 1) : am;
+          const uint32_t ar = len > 0 && rest > 0 ? am + 1 + (rest >> 1) : am;
+          vm[r] = codes[am];
+          vl[r] = codes[al];
+          vr[r] = codes[ar];
+        }
+        // all 3R loaded values pass through one empty asm: every load issues
+        // before the first wait.  Left alone, the compiler turns "select
+        // between two loaded values" into "load at the selected address"
+        // (which waits for the first step before the second step's load) and
+        // interleaves the waits of one search with the loads of the next.
+        static_assert(R == 8, "the asm below names 8 searches");
+#define GS_P3(r) "+v"(vm[r]), "+v"(vl[r]), "+v"(vr[r])
+        asm("" : GS_P3(0), GS_P3(1), GS_P3(2), GS_P3(3), GS_P3(4), GS_P3(5), GS_P3(6), GS_P3(7));
+#undef GS_P3
+        // both steps as selects (a finished search keeps len 0 and its first)
         more = false;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          if (slen[r] > 0) {
-            const uint32_t half = slen[r] >> 1;
-            if (codes[sfirst[r] + half] < stgt[r]) {
-              sfirst[r] += half + 1;
-              slen[r] -= half + 1;
-            } else {
-              slen[r] = half;
-            }
-            more |= slen[r] > 0;
-          }
+          const uint32_t len = slen[r], half = len >> 1;
+          const bool right = len > 0 && vm[r] < stgt[r];
+          uint32_t f = right ? sfirst[r] + half + 1 : sfirst[r];
+          uint32_t l = right ? len - half - 1 : half;
+          const uint32_t v2 = right ? vr[r] : vl[r];
+          const uint32_t h2 = l >> 1;
+          const bool right2 = l > 0 && v2 < stgt[r];
+          f = right2 ? f + h2 + 1 : f;
+          l = right2 ? l - h2 - 1 : h2;
+          sfirst[r] = f;
+          slen[r] = l;
+          more |= l > 0;
         }
       }
 #pragma unroll
