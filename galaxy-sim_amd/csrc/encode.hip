@@ -11,8 +11,10 @@
 //                 Number math), the per-level node sizes f32(size * 2^-d), and
 //                 the slot's root node covering [0, n); clears the slot's
 //                 digit histograms.
-//   encode_kernel q = min(ToInt32((p - min) * s), 1023) per axis, again in
-//                 soft-double, code = part1by2(qx) | qy<<1 | qz<<2; writes the
+//   encode_kernel q = min(ToInt32((p - min) * s), 1023) per axis in binary64
+//                 (hardware operations, each correctly rounded as in JS; the
+//                 soft-double ToInt32 wrap outside the int32 range), code =
+//                 part1by2(qx) | qy<<1 | qz<<2; writes the
 //                 sort's (key, slot-local index) input, padding keys
 //                 0xffffffff (sorted last) up to the end of the slot's last
 //                 active chunk, and the slot's 4 digit histograms.
@@ -195,21 +197,43 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   }
 }
 
-__device__ __forceinline__ uint32_t quant(float p, uint64_t mn, uint64_t s) {
-  // Math.min((p - min) * s | 0, 1023); non-finite p gives NaN/Inf -> 0
+__device__ __forceinline__ uint32_t quant(float p, double mn, double s) {
+  // Math.min((p - min) * s | 0, 1023); non-finite p gives NaN/Inf -> 0.
+  // The subtraction and the product are binary64 operations with one
+  // round-to-nearest-even rounding each (no contraction; binary64 subnormals
+  // are kept), the same as the JS Number arithmetic.  p is finite, so q is
+  // finite; ToInt32 truncates, and wraps modulo 2^32 outside the int32 range
+  // (a particle with one finite coordinate beyond the box of the finite
+  // particles), which the integer routine reproduces.
   if (!__builtin_isfinite(p)) return 0;
-  const uint64_t q = sd::mul(sd::sub(sd::from_f32(__float_as_uint(p)), mn), s);
-  const int32_t v = sd::to_int32(q);
+  const double q = ((double)p - mn) * s;
+  const int32_t v = q > -2147483649.0 && q < 2147483648.0
+                        ? (int32_t)q
+                        : sd::to_int32((uint64_t)__double_as_longlong(q));
   return (uint32_t)(v < 1023 ? v : 1023);  // part1by2 keeps the low 10 bits
 }
 
-// digit histograms of one key for every pass, into LDS hist[pass * radix + digit]
+// digit histograms of one key for every pass, into LDS hist[pass * radix + digit].
+// Keys of neighbouring particles mostly share their high digits (the
+// particles are in the previous step's Morton order): when every active lane
+// of the wavefront has the same digit, one lane adds the count, instead of
+// 64 atomics to one LDS word.
 template <int BITS>
 __device__ __forceinline__ void count_digits(uint32_t* hist, uint32_t code) {
   constexpr uint32_t RADIX = 1u << BITS;
+  const uint64_t live = __builtin_amdgcn_read_exec();
 #pragma unroll
-  for (int ps = 0; ps < gs_sort_passes(BITS); ps++)
-    atomicAdd(&hist[ps * RADIX + ((code >> (ps * BITS)) & (RADIX - 1))], 1u);
+  for (int ps = 0; ps < gs_sort_passes(BITS); ps++) {
+    const uint32_t d = (code >> (ps * BITS)) & (RADIX - 1);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(d == d0) == live) {
+      if (__builtin_amdgcn_mbcnt_hi((uint32_t)(live >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((uint32_t)live, 0u)) == 0u)
+        atomicAdd(&hist[ps * RADIX + d0], (uint32_t)__popcll(live));
+    } else {
+      atomicAdd(&hist[ps * RADIX + d], 1u);
+    }
+  }
 }
 
 template <int BITS>
@@ -237,8 +261,10 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
   __shared__ uint32_t hist[HW];
   for (int k = threadIdx.x; k < HW; k += blockDim.x) hist[k] = 0;
   __syncthreads();
-  const uint64_t mx = sc->min_bits[0], my = sc->min_bits[1], mz = sc->min_bits[2];
-  const uint64_t s = sc->s_bits;
+  const double mx = __longlong_as_double((long long)sc->min_bits[0]);
+  const double my = __longlong_as_double((long long)sc->min_bits[1]);
+  const double mz = __longlong_as_double((long long)sc->min_bits[2]);
+  const double s = __longlong_as_double((long long)sc->s_bits);
   const bool empty = sc->nfinite == 0;
   const uint32_t end = min(ct.y + (uint32_t)GS_CHUNK, sl.cap);
   for (uint32_t i = ct.y + threadIdx.x; i < end; i += blockDim.x) {
