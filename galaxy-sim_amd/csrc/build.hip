@@ -70,6 +70,10 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
   // count leave without touching the ticket counter (tickets keep the
   // look-back order; the blocks that stay still take every tile)
   if (blockIdx.x >= total_tiles) return;
+  // when the grid covers every tile, the blocks that stay are exactly
+  // total_tiles and their first tickets are 0 .. total_tiles - 1: each block
+  // builds one tile and leaves without asking for another
+  const bool one_tile = total_tiles <= gridDim.x;
   const int shift = 27 - 3 * d;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 
@@ -315,6 +319,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
                                      (int)gs_meta_internal((uint32_t)d, mask, nchild), skip);
       }
     }
+    if (one_tile) break;
     __syncthreads();
   }
 }

@@ -10,7 +10,8 @@
 // Launch plan: one pass over every leaf of every level (a leaf only reads
 // particles; leaves larger than a wavefront are queued and then summed by a
 // wavefront each), then one launch per internal level, deepest first, each
-// thread summing one node's children (contiguous, octant order).  Kernel boundaries
+// thread summing one node's children (contiguous, octant order); the four
+// smallest levels share one launch of a block per slot.  Kernel boundaries
 // order the levels; no device-wide fences or atomics are needed, which on a
 // multi-XCD GPU (8 non-coherent L2s) is far cheaper than per-node
 // release/acquire.  Node counts per level come from device scalars, so the
@@ -156,6 +157,46 @@ __global__ __launch_bounds__(256) void refresh_big_kernel(const float4* __restri
   }
 }
 
+// one internal node (a leaf is left alone): FP64 sum over its children in
+// octant order
+__device__ __forceinline__ void refresh_node(NodeRec* __restrict__ nodes, uint32_t idx,
+                                             uint32_t* massless) {
+  const int4 inf = nodes[idx].info;
+  if (gs_meta_is_leaf((uint32_t)inf.z)) return;
+  double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  const uint32_t nc = gs_meta_nc((uint32_t)inf.z);
+  const uint32_t fc = gs_node(inf.y) + 1u - nc;  // y = last child
+  for (uint32_t c = fc; c < fc + nc; c++) {
+    const float4 q = nodes[c].cm;
+    const double qm = (double)q.w;
+    m = __dadd_rn(m, qm);
+    cx = __dadd_rn(cx, __dmul_rn((double)q.x, qm));
+    cy = __dadd_rn(cy, __dmul_rn((double)q.y, qm));
+    cz = __dadd_rn(cz, __dmul_rn((double)q.z, qm));
+  }
+  store_com(nodes, idx, m, cx, cy, cz, massless);
+}
+
+// levels below GS_REFRESH_TOP hold at most 8^d nodes per slot: one block
+// per slot refreshes them all, deepest first, a block barrier ordering the
+// levels (the children of level d were written by this block or by an
+// earlier launch)
+constexpr int GS_REFRESH_TOP = 4;
+
+__global__ __launch_bounds__(256) void refresh_top_kernel(NodeRec* __restrict__ pool,
+                                                          const DevSlot* __restrict__ slots,
+                                                          const DevScalars* __restrict__ scs,
+                                                          uint32_t* __restrict__ massless) {
+  const uint32_t slot = blockIdx.x;
+  NodeRec* nodes = pool + slots[slot].noff;
+  for (int d = GS_REFRESH_TOP - 1; d >= 0; d--) {
+    const uint32_t start = scs[slot].level_start[d], count = scs[slot].level_count[d];
+    for (uint32_t f = threadIdx.x; f < count; f += blockDim.x)
+      refresh_node(nodes, start + f, massless + slot);
+    __syncthreads();
+  }
+}
+
 // internal nodes of one level of every slot: FP64 sum over children in
 // octant order
 __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict__ pool,
@@ -175,20 +216,7 @@ __global__ __launch_bounds__(256) void refresh_level_kernel(NodeRec* __restrict_
     const uint32_t slot = gs_find_slot(s_pre, nslots, f);
     NodeRec* nodes = pool + slots[slot].noff;
     const uint32_t idx = scs[slot].level_start[d] + (f - s_pre[slot]);
-    const int4 inf = nodes[idx].info;
-    if (gs_meta_is_leaf((uint32_t)inf.z)) continue;
-    double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    const uint32_t nc = gs_meta_nc((uint32_t)inf.z);
-    const uint32_t fc = gs_node(inf.y) + 1u - nc;  // y = last child
-    for (uint32_t c = fc; c < fc + nc; c++) {
-      const float4 q = nodes[c].cm;
-      const double qm = (double)q.w;
-      m = __dadd_rn(m, qm);
-      cx = __dadd_rn(cx, __dmul_rn((double)q.x, qm));
-      cy = __dadd_rn(cy, __dmul_rn((double)q.y, qm));
-      cz = __dadd_rn(cz, __dmul_rn((double)q.z, qm));
-    }
-    store_com(nodes, idx, m, cx, cy, cz, massless + slot);
+    refresh_node(nodes, idx, massless + slot);
   }
 }
 
@@ -212,8 +240,9 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   // internal nodes exist only at depths 0..9; a level's node count is bounded
-  // by nslots * 8^d and by the pools
-  for (int d = GS_MAX_LEVEL - 1; d >= 0; d--) {
+  // by nslots * 8^d and by the pools.  The levels below GS_REFRESH_TOP share
+  // one launch.
+  for (int d = GS_MAX_LEVEL - 1; d >= GS_REFRESH_TOP; d--) {
     uint64_t bound = (uint64_t)s->nslots << (3 * d);
     if (bound > s->ntot) bound = s->ntot;
     uint32_t gl = (uint32_t)((bound + 255) / 256);
@@ -223,5 +252,6 @@ hipError_t gs_launch_refresh(gs_sim* s, const float4* posm) {
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  refresh_top_kernel<<<s->nslots, 256, 0, s->stream>>>(s->nodes, s->slots, s->sc, s->massless);
+  return hipGetLastError();
 }
