@@ -3,9 +3,12 @@
 // The Morton quantization of mortonTree.js:53-58 runs in JS Numbers (IEEE
 // double): size = max(dx,dy,dz)*1.0001 + 1e-9, s = 1024/size and
 // q = ToInt32((p - min) * s).  Reproducing those codes bit-for-bit needs the
-// exact double roundings; this header performs them with 64-bit integer ops
-// so the encode kernel uses no FP64 units.  The same code compiles for the
-// host (g++) so tests can check it against native double arithmetic.
+// exact double roundings; this header performs them with 64-bit integer ops.
+// The root kernel's one lane computes the scale with it; the encode kernel's
+// per-particle product uses binary64 hardware operations (equally correctly
+// rounded) and this header only for ToInt32 of out-of-range values.  The
+// same code compiles for the host (g++) so tests can check it against native
+// double arithmetic.
 //
 // Values are passed as raw uint64 bit patterns.  Inputs may be zero, normal or
 // subnormal; results round correctly, including subnormal results and
