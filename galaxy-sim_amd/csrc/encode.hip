@@ -7,8 +7,9 @@
 //                 sort tickets and onesweep status words.
 //   root_kernel   one block per slot: reduces the slot's partials, then one
 //                 lane evaluates size = max(dx,dy,dz)*1.0001 + 1e-9 and
-//                 s = 1024/size in integer soft-double (bit-exact with the JS
-//                 Number math), the per-level node sizes f32(size * 2^-d), and
+//                 s = 1024/size in binary64 (correctly rounded operations,
+//                 bit-exact with the JS Number math), the per-level node sizes
+//                 f32(size * 2^-d), and
 //                 the slot's root node covering [0, n); clears the slot's
 //                 digit histograms.
 //   encode_kernel q = min(ToInt32((p - min) * s), 1023) per axis in binary64
@@ -155,31 +156,32 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
     for (int j = 0; j < 3; j++) sc->min_bits[j] = 0x7ff0000000000000ull;
     sc->s_bits = 0x8000000000000000ull;
   } else {
-    uint64_t d[3];
+    // binary64 operations, each correctly rounded (no contraction), as the
+    // JS Number arithmetic: d = max - min per axis (>= +0), size =
+    // max(d) * 1.0001 + 1e-9, s = 1024 / size
+    double d[3];
     for (int j = 0; j < 3; j++) {
-      const uint64_t mn = sd::from_f32(__float_as_uint(r[j]));
-      const uint64_t mx = sd::from_f32(__float_as_uint(r[j + 3]));
-      sc->min_bits[j] = mn;
-      d[j] = sd::sub(mx, mn);  // >= +0
+      const double mn = (double)r[j];
+      sc->min_bits[j] = (uint64_t)__double_as_longlong(mn);
+      d[j] = (double)r[j + 3] - mn;
     }
-    uint64_t m = d[0];
-    if (sd::lt(m, d[1])) m = d[1];
-    if (sd::lt(m, d[2])) m = d[2];
-    const uint64_t k10001 = 0x3FF00068DB8BAC71ull;  // 1.0001
-    const uint64_t k1em9 = 0x3E112E0BE826D695ull;   // 1e-9
-    size = sd::add(sd::mul(m, k10001), k1em9);
-    sc->s_bits = sd::div(0x4090000000000000ull /* 1024.0 */, size);
+    double m = d[0];
+    if (m < d[1]) m = d[1];
+    if (m < d[2]) m = d[2];
+    const double sz = m * 1.0001 + 1e-9;
+    size = (uint64_t)__double_as_longlong(sz);
+    sc->s_bits = (uint64_t)__double_as_longlong(1024.0 / sz);
   }
   sc->size_bits = size;
   // node sizes per depth: f32(rootSize * 0.5^d) (mortonTree.js:124,161)
-  uint64_t sz = size;
+  double sz = __longlong_as_double((long long)size);
   float min_s2 = INFINITY;
   for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
-    const float f = __uint_as_float(sd::to_f32(sz));
+    const float f = (float)sz;
     sc->level_size[dpt] = f;
     // nodes exist at depths 0..GS_MAX_LEVEL only; fminf drops a NaN
     if (dpt <= GS_MAX_LEVEL) min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));
-    if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
+    sz = sz * 0.5;
   }
   sc->min_s2 = min_s2;
   // level 0 = the root, covering [0, n); an empty slot has no tree
