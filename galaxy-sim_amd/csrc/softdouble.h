@@ -4,9 +4,8 @@
 // double): size = max(dx,dy,dz)*1.0001 + 1e-9, s = 1024/size and
 // q = ToInt32((p - min) * s).  Reproducing those codes bit-for-bit needs the
 // exact double roundings; this header performs them with 64-bit integer ops.
-// The root kernel's one lane computes the scale with it; the encode kernel's
-// per-particle product uses binary64 hardware operations (equally correctly
-// rounded) and this header only for ToInt32 of out-of-range values.  The
+// The kernels compute with binary64 hardware operations (equally correctly
+// rounded) and use this header for ToInt32 of out-of-range values only.  The
 // same code compiles for the host (g++) so tests can check it against native
 // double arithmetic.
 //
