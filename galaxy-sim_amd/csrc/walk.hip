@@ -118,6 +118,9 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
   const float selfpot = FASTLEAF ? p.w * __builtin_amdgcn_rsqf(eps2) : 0.f;
   int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;  // node link (record byte offset)
+  // the wavefront's execution mask in the node loop (every lane of the
+  // block's wavefronts; the loop holds no divergent branch around a visit)
+  const uint64_t full = __builtin_amdgcn_read_exec();
   // the kernel's own scalar loads (parameters, slot) complete here, so the
   // loop below carries no wait for them
   __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
@@ -149,8 +152,6 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       asm("s_lshl_b32 %0, %1, 1" : "=s"(mbits) : "s"(__float_as_uint(c.w)));
       dtest = mbits != 0u ? (int32_t)depth : 64;
     }
-    // the lanes that opened every ancestor, as a wave mask (one compare)
-    const uint64_t act = __builtin_amdgcn_ballot_w64(dtest <= alim);
     const float s2 = __int_as_float(inf.x);  // the record holds size * size
     const bool leaf = gs_meta_is_leaf(meta);
     const uint32_t first = (uint32_t)inf.y;  // leaf: first member; internal: last child
@@ -162,16 +163,52 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
     // compare of non-negative float bits on the scalar unit, made only when
     // some node of the tree is that small (GENERIC).  Every other active lane
     // opens the node.
-    uint64_t mono_raw = __builtin_amdgcn_ballot_w64(s2 < theta2 * d2);
-    if (GENERIC && (uint32_t)inf.x < qbits) mono_raw = ~0ull;
-    const uint64_t mono_mask = act & mono_raw;
-    const uint64_t need_mask = act & ~mono_raw;
-    if (__builtin_amdgcn_inverse_ballot_w64(mono_mask)) {
-      if (EXACT)
-        interact_exact(G * c.w, rx, ry, rz, d2, ax, ay, az, pot);
-      else
-        interact_fast(c.w, rx, ry, rz, d2, ax, ay, az, pot);
-      if (COUNT) cnt++;
+    uint64_t act, need_mask;
+    if (!EXACT && !COUNT && !GENERIC) {
+      // the two compares write the execution mask themselves (v_cmpx writes
+      // its result to an SGPR pair and to EXEC): EXEC = active lanes, then
+      // EXEC = active lanes accepting the monopole, whose interaction
+      // (interact_fast, same instructions and order) runs under that mask;
+      // EXEC is restored before the block ends.  Saves the scalar AND and
+      // save-exec of the compiled branch on every visit.
+      const float t = theta2 * d2;
+      uint64_t mono;
+      float inv, mi;
+      asm volatile(
+          "v_cmpx_le_i32_e64 %[act], %[dep], %[alim]\n\t"
+          "v_cmpx_lt_f32_e64 %[mono], %[s2], %[t]\n\t"
+          "s_cbranch_execz 1f\n\t"
+          "v_rsq_f32_e32 %[inv], %[d2]\n\t"
+          "s_nop 0\n\t"
+          "v_mul_f32_e32 %[mi], %[m], %[inv]\n\t"
+          "v_mul_f32_e32 %[inv], %[inv], %[inv]\n\t"
+          "v_mul_f32_e32 %[inv], %[mi], %[inv]\n\t"
+          "v_fmac_f32_e32 %[ax], %[rx], %[inv]\n\t"
+          "v_fmac_f32_e32 %[ay], %[ry], %[inv]\n\t"
+          "v_fmac_f32_e32 %[az], %[rz], %[inv]\n\t"
+          "v_sub_f32_e32 %[pot], %[pot], %[mi]\n"
+          "1:\n\t"
+          "s_mov_b64 exec, %[full]"
+          : [act] "=&s"(act), [mono] "=&s"(mono), [inv] "=&v"(inv), [mi] "=&v"(mi),
+            [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [pot] "+v"(pot)
+          : [dep] "s"(dtest), [alim] "v"(alim), [s2] "s"(s2), [t] "v"(t), [d2] "v"(d2),
+            [m] "s"(c.w), [rx] "v"(rx), [ry] "v"(ry), [rz] "v"(rz), [full] "s"(full)
+          : "scc");
+      need_mask = act & ~mono;
+    } else {
+      // the lanes that opened every ancestor, as a wave mask (one compare)
+      act = __builtin_amdgcn_ballot_w64(dtest <= alim);
+      uint64_t mono_raw = __builtin_amdgcn_ballot_w64(s2 < theta2 * d2);
+      if (GENERIC && (uint32_t)inf.x < qbits) mono_raw = ~0ull;
+      const uint64_t mono_mask = act & mono_raw;
+      need_mask = act & ~mono_raw;
+      if (__builtin_amdgcn_inverse_ballot_w64(mono_mask)) {
+        if (EXACT)
+          interact_exact(G * c.w, rx, ry, rz, d2, ax, ay, az, pot);
+        else
+          interact_fast(c.w, rx, ry, rz, d2, ax, ay, az, pot);
+        if (COUNT) cnt++;
+      }
     }
     // lanes that descend into the node's children (none for a leaf)
     uint64_t desc_mask = need_mask;
