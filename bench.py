@@ -438,6 +438,9 @@ def main():
             "hbm_gbs": (fetch + write) / (out["roofline"]["walk_avg_ms"] * 1e-3) / 1e9,
             "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per walk dispatch, scaled by a "
                       f"{COPY_PROBE_MIB} MiB device copy"}
+        gbs = out["roofline"]["traffic_detail"]["hbm_gbs"]
+        out["roofline"]["hbm"]["measured_gbs"] = gbs
+        out["roofline"]["hbm"]["measured_frac"] = gbs / HBM_PEAK_GBS
         out["roofline"]["l2_hit_rate"] = traffic["L2"]["hit_rate"] if "L2" in traffic else None
         if "L2" in traffic:
             out["roofline"]["l2_detail"] = {
@@ -533,6 +536,13 @@ def bench_headline(args):
             "interactions_per_step": inter, "flop_per_interaction": WALK_FLOP_PER_INTERACTION,
             "walk_avg_ms": walk_avg_ms,
             "walk_algorithmic_bytes": (68 + 64 * nu) * n,
+            # the walk against the HBM roofline too: its algorithmic bytes per
+            # launch over its launch time (the rocprof-measured rate is added
+            # when the counter passes run)
+            "hbm": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                    "achieved": (68 + 64 * nu) * n / (walk_avg_ms * 1e-3) / 1e9,
+                    "frac": (68 + 64 * nu) * n / (walk_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "measured_gbs": None, "measured_frac": None},
         },
         "step_hbm": {
             "bytes_per_particle": 236 + 160 * nu, "nu": nu,
