@@ -262,8 +262,13 @@ void launch_walk_step(gs_sim* s) {
 void mark_sorted(gs_sim* s) {
   for (auto& h : s->hs) {
     h.have_tree = h.d.n > 0;
+    h.have_com = false;  // the build writes the info words only
     h.have_force = false;
   }
+}
+
+void mark_refreshed(gs_sim* s) {
+  for (auto& h : s->hs) h.have_com = h.d.n > 0 && h.have_tree;
 }
 
 void mark_stepped(gs_sim* s) {
@@ -271,6 +276,12 @@ void mark_stepped(gs_sim* s) {
     h.have_step = h.d.n > 0;
     h.have_force = false;
   }
+}
+
+// the walk reads node COMs: every occupied slot needs a tree whose COM words
+// were filled (refresh, tree upload or a full step)
+void need_com(gs_sim* s, const char* msg) {
+  for (auto& h : s->hs) need(!h.d.n || (h.have_tree && h.have_com), GS_E_STATE, msg);
 }
 
 bool use_graph(const gs_sim* s) { return !(s->flags & GS_FLAG_NO_GRAPH); }
@@ -340,6 +351,7 @@ void run_step(gs_sim* s, bool sort) {
     launch_walk_step(s);
   }
   if (sort) mark_sorted(s);
+  mark_refreshed(s);
   mark_stepped(s);
 }
 
@@ -523,7 +535,7 @@ void upload_particles(gs_sim* s, uint32_t slot, const float* particles, uint64_t
   push_slot(s, slot);
   push_params(s, slot);
   sync(s);
-  h.have_tree = h.have_force = h.have_step = h.have_codes = false;
+  h.have_tree = h.have_force = h.have_step = h.have_codes = h.have_com = false;
 }
 
 // Stellar records are allocated on first upload; step graphs captured
@@ -783,6 +795,22 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
         }
       }
     }
+    // levelRanges (mortonTree.js:158): [start, count] per depth, ascending
+    // and contiguous from node 0, covering every node, each node at the depth
+    // of its range; the refresh walks these ranges without further bounds
+    if (num_levels > 0) {
+      uint64_t next = 0;
+      for (uint32_t k = 0; k < num_levels; k++) {
+        const uint64_t start = level_ranges[2 * k], count = level_ranges[2 * k + 1];
+        need(start == next && count <= num_nodes - start, GS_E_INVALID,
+             "level ranges must be contiguous from node 0 and inside the tree");
+        for (uint64_t i = start; i < start + count; i++)
+          need((uint32_t)links[i].y == k, GS_E_INVALID,
+               "a node's depth differs from the level range holding it");
+        next = start + count;
+      }
+      need(next == num_nodes, GS_E_INVALID, "level ranges must cover every node");
+    }
     if (!s->node64) dalloc(&s->node64, (size_t)s->max_ncap * 16, "node staging");
     check(hipMemcpyAsync(s->node64, nodes, num_nodes * 64, hipMemcpyHostToDevice, s->stream),
           "nodes");
@@ -806,6 +834,7 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
     check(e, "import");
     check_status(s);  // non-contiguous children are rejected here
     h.have_tree = true;
+    h.have_com = true;  // the uploaded words carry the caller's COMs
     h.have_force = false;
   });
 }
@@ -818,7 +847,7 @@ int gs_encode(gs_sim* s) {
     check(gs_launch_encode(s, s->posm[s->cur], true), "encode");
     for (auto& h : s->hs) {
       h.have_codes = h.d.n > 0;
-      h.have_tree = false;  // the root node slot was rewritten
+      h.have_tree = h.have_com = false;  // the root node slot was rewritten
     }
   });
 }
@@ -835,12 +864,13 @@ int gs_refresh(gs_sim* s) {
   return guarded(s, [&] {
     for (auto& h : s->hs) need(!h.d.n || h.have_tree, GS_E_STATE, "refresh before any sort");
     launch_refresh(s);
+    mark_refreshed(s);
   });
 }
 
 int gs_force(gs_sim* s) {
   return guarded(s, [&] {
-    for (auto& h : s->hs) need(!h.d.n || h.have_tree, GS_E_STATE, "force before any sort");
+    need_com(s, "force needs a tree with COMs (sort, then refresh or upload a tree)");
     Stage t(s, GS_T_WALK);
     check(gs_launch_walk(s, false, false), "walk");
     for (auto& h : s->hs) h.have_force = h.d.n > 0;
@@ -857,6 +887,20 @@ int gs_step(gs_sim* s, uint32_t nsteps, uint32_t sort_every) {
       run_step(s, sort);
       s->step_count++;
     }
+  });
+}
+
+int gs_walk_step(gs_sim* s) {
+  return guarded(s, [&] {
+    need_com(s, "step needs a tree with COMs (refresh, upload a tree or gs_step first)");
+    if (use_graph(s)) {
+      Stage w(s, GS_T_WALK);
+      check(hipGraphLaunch(step_graph(s, false, 2), s->stream), "graph launch");
+      s->cur ^= 1;
+    } else {
+      launch_walk_step(s);
+    }
+    mark_stepped(s);
   });
 }
 
@@ -945,6 +989,7 @@ int gs_slot_download_nodes(gs_sim* s, uint32_t slot, uint32_t* dst, uint64_t cap
   return guarded(s, [&] {
     const HostSlot& hs = slot_of(s, slot);
     need(hs.have_tree, GS_E_STATE, "no tree yet");
+    need(hs.have_com, GS_E_STATE, "node COMs not filled since the last sort (refresh first)");
     check_status(s);
     const uint32_t nn = read_scalars(s, slot).num_nodes;
     if (num_nodes) *num_nodes = nn;
@@ -1075,7 +1120,7 @@ int gs_stage_times(gs_sim* s, double ms[GS_NT], uint64_t counts[GS_NT], int rese
 
 int gs_count_interactions(gs_sim* s, uint64_t* total, uint32_t* per_particle) {
   return guarded(s, [&] {
-    for (auto& h : s->hs) need(!h.d.n || h.have_tree, GS_E_STATE, "no tree yet");
+    need_com(s, "counting needs a tree with COMs (sort, then refresh or upload a tree)");
     uint64_t zero = 0;
     check(hipMemcpyAsync(&s->batch->interactions, &zero, 8, hipMemcpyHostToDevice, s->stream),
           "zero");
@@ -1120,7 +1165,7 @@ int gs_compute_morton_codes(gs_sim* s, const float* pos, uint32_t n, uint32_t st
     check(hipMemcpyAsync(&sz, &s->sc->size_bits, 8, hipMemcpyDeviceToHost, s->stream), "size");
     sync(s);
     if (root_size) memcpy(root_size, &sz, 8);
-    h.have_tree = false;
+    h.have_tree = h.have_com = false;
     h.have_codes = true;
   });
 }
@@ -1141,7 +1186,7 @@ int gs_radix_sort_indices(gs_sim* s, const uint32_t* codes, uint32_t n, uint32_t
                          hipMemcpyDeviceToHost, s->stream),
           "perm");
     sync(s);
-    h.have_tree = false;
+    h.have_tree = h.have_com = false;
   });
 }
 
@@ -1197,6 +1242,7 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     check(gs_launch_build(s), "build");
     check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
     h.have_tree = true;
+    h.have_com = true;
     check_status(s);
     const DevScalars r = read_scalars(s, 0);
     if (num_nodes) *num_nodes = r.num_nodes;

@@ -226,6 +226,9 @@ struct HostSlot {
   DevSlot d;
   bool have_params = false, have_tree = false, have_codes = false, have_force = false,
        have_step = false;
+  // node COM/mass words hold sums of particles (a refresh, a tree upload or a
+  // step ran since the last topology build, which writes only the info words)
+  bool have_com = false;
   bool fixed_max_nodes = false;  // else the leaf rule's pool follows n: 2n + 4096
 };
 

@@ -102,9 +102,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // One particle per lane, the lanes of a wavefront walking one node sequence.
-// FASTLEAF (default path with eps > 0): leaf members run without the j == i
-// test, since that pair adds an exact zero to the acceleration (r = 0, f
-// finite); the self potential -m_i / eps is taken back once per leaf.  The
+// FASTLEAF (default path with eps > 0 and a finite self term): leaf members
+// run without the j == i test, since that pair adds an exact zero to the
+// acceleration (r = 0, f finite); the self potential -m_i / eps is taken back
+// once per leaf.  The
 // kernel picks the instance once per block, so the node loop carries no
 // test of it.
 template <bool EXACT, bool COUNT, bool FASTLEAF, bool GENERIC>
@@ -329,12 +330,24 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   // without both tests unless the slot needs either
   const bool generic = scs[slot].min_s2 < qeps2 || massless[slot] != 0u;
 
+  // FASTLEAF adds a lane's own j == i pair as r * f with r = 0, an exact zero
+  // only while f = m rsq(eps2)^3 (and m rsq(eps2)) is finite; a huge mass or
+  // a tiny eps makes it Inf and the zero a NaN, where the reference skips the
+  // pair.  The block takes FASTLEAF only if every lane's self term is finite.
+  bool fastleaf = false;
+  if (!EXACT && !COUNT && eps2 > 0.0f) {
+    const float inv = __builtin_amdgcn_rsqf(eps2);
+    const float mi = p.w * inv;
+    const float f = mi * (inv * inv);
+    fastleaf = __syncthreads_and(__builtin_isfinite(mi) && __builtin_isfinite(f)) != 0;
+  }
+
   float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
   uint32_t cnt = 0;
 #define GS_WALK(FL, QC)                                                                      \
   walk_particle<EXACT, COUNT, FL, QC>(nodes, nsize, posm, p, i, valid, G, eps2, theta2,      \
                                       qbits, ax, ay, az, pot, dens, cnt)
-  if (!EXACT && !COUNT && eps2 > 0.0f) {
+  if (fastleaf) {
     if (generic) GS_WALK(true, true);
     else GS_WALK(true, false);
   } else {

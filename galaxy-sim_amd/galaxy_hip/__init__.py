@@ -94,6 +94,7 @@ def lib():
         "gs_refresh": (ctypes.c_int, [_vp]),
         "gs_force": (ctypes.c_int, [_vp]),
         "gs_step": (ctypes.c_int, [_vp, _u32, _u32]),
+        "gs_walk_step": (ctypes.c_int, [_vp]),
         "gs_sync": (ctypes.c_int, [_vp]),
         "gs_apply_permutation": (ctypes.c_int, [_vp, _vp, _u64]),
         "gs_upload_tree": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u32]),
@@ -245,6 +246,10 @@ class NBodyGPU:
         node layout for the current particle order."""
         words = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1)
         lr = np.ascontiguousarray(np.asarray(level_ranges, np.uint32).reshape(-1))
+        if int(num_nodes) < 1 or words.size < 16 * int(num_nodes):
+            raise ValueError("tree words must hold 16 x num_nodes uint32 values")
+        if lr.size % 2 or lr.size > 32:
+            raise ValueError("level_ranges must be at most 16 [start, count] pairs")
         self._call("gs_upload_tree", _ptr(words), int(num_nodes), _ptr(lr) if lr.size else None,
                    lr.size // 2)
 
@@ -292,6 +297,11 @@ class NBodyGPU:
     def step(self, nsteps=1, sort_every=1):
         """nsteps x (refresh + step) with a rebuild every sort_every steps."""
         self._call("gs_step", nsteps, sort_every)
+
+    def walk_step(self):
+        """step (NBodyGPU.js:321-329): walk + integrate on the tree's current
+        COMs, no refresh (encodeRefresh is refresh())."""
+        self._call("gs_walk_step")
 
     def sync(self):
         self._call("gs_sync")

@@ -113,10 +113,12 @@ export class NBodyGPU {
     addon.refresh(this.h);
   }
 
-  /** step (NBodyGPU.js:321-329): walk + halos + kick-drift, then flip. */
+  /** step (NBodyGPU.js:321-329): the `main` dispatch alone -- walk, halos,
+   * kick-drift and gas drag on the tree's current COMs, then flip.  The
+   * refresh is encodeRefresh's, as in the reference frame loop. */
   step(commandEncoder) {
     if (!this.ready) return;
-    addon.step(this.h, 1, 0);
+    addon.walkStep(this.h);
     this.currentBuffer ^= 1;
   }
 

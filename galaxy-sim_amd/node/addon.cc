@@ -238,6 +238,7 @@ SIMPLE(Encode, gs_encode)
 SIMPLE(Sort, gs_sort)
 SIMPLE(Refresh, gs_refresh)
 SIMPLE(Force, gs_force)
+SIMPLE(WalkStep, gs_walk_step)
 SIMPLE(Sync, gs_sync)
 
 napi_value Step(napi_env env, napi_callback_info info) {
@@ -404,13 +405,20 @@ napi_value ComputeMortonCodes(napi_env env, napi_callback_info info) {
   Args a;
   get_args(env, info, a);
   SIM_OR_RETURN(s, a.v[0]);
-  float* pos = typed<float>(env, a.v[1], napi_float32_array, nullptr);
+  size_t npos = 0, ncodes = 0;
+  float* pos = typed<float>(env, a.v[1], napi_float32_array, &npos);
   if (!pos) return nullptr;
   double n = 0, stride = 0;
   napi_get_value_double(env, a.v[2], &n);
   napi_get_value_double(env, a.v[3], &stride);
-  uint32_t* codes = typed<uint32_t>(env, a.v[4], napi_uint32_array, nullptr);
+  uint32_t* codes = typed<uint32_t>(env, a.v[4], napi_uint32_array, &ncodes);
   if (!codes) return nullptr;
+  // the last particle reads pos[(N - 1) * stride + 0..2]
+  if (!(n >= 1) || !(stride >= 3) || ncodes < (size_t)n ||
+      npos < ((size_t)n - 1) * (size_t)stride + 3) {
+    napi_throw_range_error(env, nullptr, "computeMortonCodes: pos or codes shorter than N");
+    return nullptr;
+  }
   double rs = 0;
   int rc = gs_compute_morton_codes(s, pos, (uint32_t)n, (uint32_t)stride, codes, &rs);
   if (rc) return throw_gs(env, s, rc, "gs_compute_morton_codes");
@@ -422,12 +430,17 @@ napi_value RadixSortIndices(napi_env env, napi_callback_info info) {
   Args a;
   get_args(env, info, a);
   SIM_OR_RETURN(s, a.v[0]);
-  uint32_t* codes = typed<uint32_t>(env, a.v[1], napi_uint32_array, nullptr);
+  size_t nc = 0, no = 0;
+  uint32_t* codes = typed<uint32_t>(env, a.v[1], napi_uint32_array, &nc);
   if (!codes) return nullptr;
   double n = 0;
   napi_get_value_double(env, a.v[2], &n);
-  uint32_t* out = typed<uint32_t>(env, a.v[3], napi_uint32_array, nullptr);
+  uint32_t* out = typed<uint32_t>(env, a.v[3], napi_uint32_array, &no);
   if (!out) return nullptr;
+  if (!(n >= 0) || nc < (size_t)n || no < (size_t)n) {
+    napi_throw_range_error(env, nullptr, "radixSortIndices: codes or indices shorter than N");
+    return nullptr;
+  }
   int rc = gs_radix_sort_indices(s, codes, (uint32_t)n, out);
   return rc ? throw_gs(env, s, rc, "gs_radix_sort_indices") : nullptr;
 }
@@ -437,13 +450,18 @@ napi_value TreeBuild(napi_env env, napi_callback_info info) {
   Args a;
   get_args(env, info, a);
   SIM_OR_RETURN(s, a.v[0]);
-  uint32_t* codes = typed<uint32_t>(env, a.v[1], napi_uint32_array, nullptr);
-  float* pos = typed<float>(env, a.v[2], napi_float32_array, nullptr);
-  float* masses = typed<float>(env, a.v[3], napi_float32_array, nullptr);
+  size_t nc = 0, np3 = 0, nm = 0;
+  uint32_t* codes = typed<uint32_t>(env, a.v[1], napi_uint32_array, &nc);
+  float* pos = typed<float>(env, a.v[2], napi_float32_array, &np3);
+  float* masses = typed<float>(env, a.v[3], napi_float32_array, &nm);
   if (!codes || !pos || !masses) return nullptr;
   double n = 0, rs = 0;
   napi_get_value_double(env, a.v[4], &n);
   napi_get_value_double(env, a.v[5], &rs);
+  if (!(n >= 0) || nc < (size_t)n || np3 < 3 * (size_t)n || nm < (size_t)n) {
+    napi_throw_range_error(env, nullptr, "treeBuild: codes, pos (3N) or masses shorter than N");
+    return nullptr;
+  }
   void* out = nullptr;
   size_t bytes = 0;
   if (napi_get_arraybuffer_info(env, a.v[6], &out, &bytes) != napi_ok) {
@@ -779,7 +797,8 @@ napi_value Init(napi_env env, napi_value exports) {
   } fns[] = {
       {"create", Create}, {"destroy", Destroy}, {"upload", Upload}, {"setParams", SetParams},
       {"updateCentralMass", UpdateCentralMass}, {"encode", Encode}, {"sort", Sort},
-      {"refresh", Refresh}, {"force", Force}, {"step", Step}, {"sync", Sync},
+      {"refresh", Refresh}, {"force", Force}, {"step", Step}, {"walkStep", WalkStep},
+      {"sync", Sync},
       {"applyPermutation", ApplyPermutation}, {"uploadTree", UploadTree},
       {"downloadParticles", DownloadParticles}, {"readPositions", ReadPositions},
       {"readPositionsAsync", ReadPositionsAsync}, {"downloadPerm", DownloadPerm},

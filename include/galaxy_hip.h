@@ -158,6 +158,14 @@ int gs_force(gs_sim* sim);
  * rebuild (gs_sort) before every step whose running index is a multiple of
  * sort_every (0 = never rebuild; a tree must exist). */
 int gs_step(gs_sim* sim, uint32_t nsteps, uint32_t sort_every);
+/* step(commandEncoder) (NBodyGPU.js:321-329): the `main` dispatch alone --
+ * walk, halos, kick-drift and gas drag on the node COMs the tree holds now,
+ * then the ping-pong flip.  It neither refreshes nor rebuilds: encodeRefresh
+ * (gs_refresh) is its own call, as in the frame loop (main.js:471-477), so a
+ * step after updateCentralMass without a refresh walks the old COMs, as the
+ * reference does.  Needs a tree whose COMs were filled (gs_refresh,
+ * gs_upload_tree or a gs_step since the last gs_sort); GS_E_STATE otherwise. */
+int gs_walk_step(gs_sim* sim);
 int gs_sync(gs_sim* sim);
 
 /* Current particle records in current (sorted) order, n x 8 f32. */

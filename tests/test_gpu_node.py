@@ -39,3 +39,29 @@ def test_node_stellar_flow():
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stderr[-2000:]
     assert "node stellar flow ok" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed on this machine")
+def test_node_step_is_the_reference_step(tmp_path):
+    """NBodyGPU.step() walks without refreshing (NBodyGPU.js:321-329): after
+    updateCentralMass and no refresh, the JS step equals the oracle's walk of
+    the unrefreshed tree, bit for bit (EXACT force path)."""
+    import numpy as np
+
+    from oracle import oracle
+
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "node", "step_semantics.mjs"),
+                        str(tmp_path)], capture_output=True, text=True, timeout=240, cwd=ROOT)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "node step semantics ok" in r.stdout
+    before = np.fromfile(tmp_path / "before.f32", np.float32)
+    nodes = np.fromfile(tmp_path / "nodes.u32", np.uint32)
+    after = np.fromfile(tmp_path / "after.f32", np.float32)
+    n = before.size // 8
+    op = oracle.pack_params(n, dt=0.005, G=1.0, epsilon=0.02, theta=0.6, gas_drag=0.25)
+    want, _, _, _, _ = oracle.walk(before, nodes, op)
+    assert np.array_equal(after.view(np.uint32), want.view(np.uint32))
+    # the central mass changed, so a refreshed tree would give other forces
+    fresh = oracle.refresh_f64(before, nodes)
+    assert not np.array_equal(fresh, nodes)

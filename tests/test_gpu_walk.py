@@ -307,3 +307,40 @@ def test_exact_rebuild_cadence():
         cur, went, _, _, _ = oracle.walk(cur, nodes, op)
     assert np.array_equal(parts.reshape(-1).view(np.uint32), cur.view(np.uint32))
     assert np.array_equal(energy.view(np.uint32), went.view(np.uint32))
+
+
+def test_walk_step_is_the_reference_step():
+    """gs_walk_step = step() (NBodyGPU.js:321-329): the walk on the tree's
+    current COMs with no refresh.  refresh + walk_step equals gs_step(1, 0);
+    after updateCentralMass without a refresh it walks the old COMs, equal to
+    the oracle's walk of the unrefreshed tree (EXACT, bit for bit).  Walking
+    or downloading nodes after a sort and before any refresh is refused."""
+    banner("walk_step semantics")
+    data = ic.make("cpu_disk_16k", n=12000)
+    n = data.shape[0]
+    base, halo, op = _params(n, theta=0.6, epsilon=0.01, gasDrag=0.25)
+    a = galaxy_hip.NBodyGPU(data, bucket=16, exact_force=True)
+    b = galaxy_hip.NBodyGPU(data, bucket=16, exact_force=True)
+    try:
+        for s in (a, b):
+            s.update_params(haloCenters=halo, **base)
+            s.sort()
+        for call in (a.walk_step, a.force, a.nodes, a.count_interactions):
+            with pytest.raises(galaxy_hip.GalaxyError):
+                call()  # no COMs since the sort
+        a.refresh()
+        a.walk_step()
+        b.step(1, sort_every=0)
+        assert np.array_equal(a.particles().view(np.uint32), b.particles().view(np.uint32))
+        assert np.array_equal(a.energy().view(np.uint32), b.energy().view(np.uint32))
+        # no refresh: the next walk uses the COMs of the refresh before the drift
+        nodes = a.nodes()
+        a.update_central_mass(3.0, 11)
+        before = a.particles()
+        a.walk_step()
+        after = a.particles()
+    finally:
+        a.destroy()
+        b.destroy()
+    want, _, _, _, _ = oracle.walk(before.reshape(-1), nodes, op)
+    assert np.array_equal(after.reshape(-1).view(np.uint32), want.view(np.uint32))
