@@ -7,14 +7,15 @@
 //                 sort tickets and onesweep status words.
 //   root_kernel   one block per slot: reduces the slot's partials, then one
 //                 lane evaluates size = max(dx,dy,dz)*1.0001 + 1e-9 and
-//                 s = 1024/size in binary64 (correctly rounded operations,
-//                 bit-exact with the JS Number math), the per-level node sizes
-//                 f32(size * 2^-d), and
+//                 s = 1024/size in integer soft-double (bit-exact with the JS
+//                 Number math; no binary64 hardware arithmetic), the per-level
+//                 node sizes f32(size * 2^-d), and
 //                 the slot's root node covering [0, n); clears the slot's
 //                 digit histograms.
-//   encode_kernel q = min(ToInt32((p - min) * s), 1023) per axis in binary64
-//                 (hardware operations, each correctly rounded as in JS; the
-//                 soft-double ToInt32 wrap outside the int32 range), code =
+//   encode_kernel q = min(ToInt32((p - min) * s), 1023) per axis, bit-exact
+//                 with binary64 (quantize.h: an f32 evaluation with a proven
+//                 error margin, integer soft-double within the margin of a
+//                 cell boundary or outside the box), code =
 //                 part1by2(qx) | qy<<1 | qz<<2; writes the
 //                 sort's (key, slot-local index) input, padding keys
 //                 0xffffffff (sorted last) up to the end of the slot's last
@@ -22,6 +23,7 @@
 // All three are HBM-streaming: 16 B read per particle for bbox, 16 B read +
 // 8 B written for encode.
 #include "gs_internal.h"
+#include "quantize.h"
 #include "softdouble.h"
 
 namespace {
@@ -153,35 +155,42 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   if (c == 0) {
     // no finite particle: minX = +Inf, size = -Inf, s = -0 -> every code is 0
     size = 0xfff0000000000000ull;
-    for (int j = 0; j < 3; j++) sc->min_bits[j] = 0x7ff0000000000000ull;
-    sc->s_bits = 0x8000000000000000ull;
-  } else {
-    // binary64 operations, each correctly rounded (no contraction), as the
-    // JS Number arithmetic: d = max - min per axis (>= +0), size =
-    // max(d) * 1.0001 + 1e-9, s = 1024 / size
-    double d[3];
     for (int j = 0; j < 3; j++) {
-      const double mn = (double)r[j];
-      sc->min_bits[j] = (uint64_t)__double_as_longlong(mn);
-      d[j] = (double)r[j + 3] - mn;
+      sc->min_bits[j] = 0x7ff0000000000000ull;
+      sc->min_f32[j] = INFINITY;
     }
-    double m = d[0];
-    if (m < d[1]) m = d[1];
-    if (m < d[2]) m = d[2];
-    const double sz = m * 1.0001 + 1e-9;
-    size = (uint64_t)__double_as_longlong(sz);
-    sc->s_bits = (uint64_t)__double_as_longlong(1024.0 / sz);
+    sc->s_bits = 0x8000000000000000ull;
+    sc->s32 = -0.0f;
+  } else {
+    // the JS Number arithmetic in integer soft-double, each operation
+    // correctly rounded: d = max - min per axis (>= +0), size =
+    // max(d) * 1.0001 + 1e-9, s = 1024 / size
+    uint64_t d[3];
+    for (int j = 0; j < 3; j++) {
+      const uint64_t mn = sd::from_f32(__float_as_uint(r[j]));
+      sc->min_bits[j] = mn;
+      sc->min_f32[j] = r[j];
+      d[j] = sd::sub(sd::from_f32(__float_as_uint(r[j + 3])), mn);
+    }
+    uint64_t m = d[0];
+    if (sd::lt(m, d[1])) m = d[1];
+    if (sd::lt(m, d[2])) m = d[2];
+    const uint64_t k10001 = 0x3FF00068DB8BAC71ull;  // 1.0001
+    const uint64_t k1em9 = 0x3E112E0BE826D695ull;   // 1e-9
+    size = sd::add(sd::mul(m, k10001), k1em9);
+    sc->s_bits = sd::div(0x4090000000000000ull /* 1024.0 */, size);
+    sc->s32 = __uint_as_float(sd::to_f32(sc->s_bits));
   }
   sc->size_bits = size;
   // node sizes per depth: f32(rootSize * 0.5^d) (mortonTree.js:124,161)
-  double sz = __longlong_as_double((long long)size);
+  uint64_t sz = size;
   float min_s2 = INFINITY;
   for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
-    const float f = (float)sz;
+    const float f = __uint_as_float(sd::to_f32(sz));
     sc->level_size[dpt] = f;
     // nodes exist at depths 0..GS_MAX_LEVEL only; fminf drops a NaN
     if (dpt <= GS_MAX_LEVEL) min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));
-    sz = sz * 0.5;
+    if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
   }
   sc->min_s2 = min_s2;
   // level 0 = the root, covering [0, n); an empty slot has no tree
@@ -197,22 +206,6 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
     nodes[sl.noff].info = make_int4(gs_s2_bits(sc->level_size[0]), 0,
                                     (int)gs_meta_open(0, sl.n), -1);
   }
-}
-
-__device__ __forceinline__ uint32_t quant(float p, double mn, double s) {
-  // Math.min((p - min) * s | 0, 1023); non-finite p gives NaN/Inf -> 0.
-  // The subtraction and the product are binary64 operations with one
-  // round-to-nearest-even rounding each (no contraction; binary64 subnormals
-  // are kept), the same as the JS Number arithmetic.  p is finite, so q is
-  // finite; ToInt32 truncates, and wraps modulo 2^32 outside the int32 range
-  // (a particle with one finite coordinate beyond the box of the finite
-  // particles), which the integer routine reproduces.
-  if (!__builtin_isfinite(p)) return 0;
-  const double q = ((double)p - mn) * s;
-  const int32_t v = q > -2147483649.0 && q < 2147483648.0
-                        ? (int32_t)q
-                        : sd::to_int32((uint64_t)__double_as_longlong(q));
-  return (uint32_t)(v < 1023 ? v : 1023);  // part1by2 keeps the low 10 bits
 }
 
 // digit histograms of one key for every pass, into LDS hist[pass * radix + digit].
@@ -263,10 +256,9 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
   __shared__ uint32_t hist[HW];
   for (int k = threadIdx.x; k < HW; k += blockDim.x) hist[k] = 0;
   __syncthreads();
-  const double mx = __longlong_as_double((long long)sc->min_bits[0]);
-  const double my = __longlong_as_double((long long)sc->min_bits[1]);
-  const double mz = __longlong_as_double((long long)sc->min_bits[2]);
-  const double s = __longlong_as_double((long long)sc->s_bits);
+  const uint64_t mxb = sc->min_bits[0], myb = sc->min_bits[1], mzb = sc->min_bits[2];
+  const uint64_t sb = sc->s_bits;
+  const float mx = sc->min_f32[0], my = sc->min_f32[1], mz = sc->min_f32[2], s32 = sc->s32;
   const bool empty = sc->nfinite == 0;
   const uint32_t end = min(ct.y + (uint32_t)GS_CHUNK, sl.cap);
   for (uint32_t i = ct.y + threadIdx.x; i < end; i += blockDim.x) {
@@ -275,8 +267,9 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
       const float4 p = posm[sl.poff + i];
       code = 0;
       if (!empty)
-        code = part1by2(quant(p.x, mx, s)) | (part1by2(quant(p.y, my, s)) << 1) |
-               (part1by2(quant(p.z, mz, s)) << 2);
+        code = part1by2(gsq::quantize(p.x, mx, s32, mxb, sb)) |
+               (part1by2(gsq::quantize(p.y, my, s32, myb, sb)) << 1) |
+               (part1by2(gsq::quantize(p.z, mz, s32, mzb, sb)) << 2);
       if (KEEP_RAW) raw[sl.poff + i] = code;
     }
     keys[sl.poff + i] = code;

@@ -108,6 +108,8 @@ struct DevScalars {
   uint64_t min_bits[3];  // bbox minimum (f32 widened to binary64 bits)
   uint64_t s_bits;       // 1024 / size  (binary64 bits)
   uint64_t size_bits;    // rootSize     (binary64 bits)
+  float min_f32[3];      // bbox minimum (f32), for the encode's f32 fast path
+  float s32;             // f32(1024 / size)
   uint32_t nfinite;      // particles that entered the bbox
   uint32_t num_nodes;
   uint32_t num_levels;

@@ -51,6 +51,18 @@ def test_softdouble_matches_native_double(tmp_path):
     assert "bad=0" in r.stdout
 
 
+def test_quantize_matches_native_double(tmp_path):
+    """The encode's quantization (f32 fast path + soft-double exact path,
+    csrc/quantize.h) equals binary64 evaluation of mortonTree.js:53-58 on
+    boxes of many scales, cell-boundary and out-of-box coordinates."""
+    exe = tmp_path / "qcheck"
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "native", "quant_check.cpp")], check=True)
+    r = subprocess.run([str(exe), "100000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    assert "bad=0" in r.stdout
+
+
 def test_initial_conditions_deterministic():
     from galaxy_hip import ic
     a = ic.make("cpu_disk_16k")

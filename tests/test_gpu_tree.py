@@ -190,6 +190,26 @@ def test_host_tree_upload_flow_matches_device_sort():
         bad[8:16] = 0
         with pytest.raises(galaxy_hip.GalaxyError):
             a.upload_tree(bad, ref["num_nodes"], ref["level_ranges"])
+        # level ranges the refresh would follow out of the tree, or that do not
+        # match the nodes' depths, are refused before any device work
+        lr = [list(r) for r in ref["level_ranges"]]
+        nn = ref["num_nodes"]
+        past = [r[:] for r in lr]
+        past[-1][1] += 5  # runs past num_nodes
+        gap = [r[:] for r in lr]
+        gap[1][0] += 1  # not contiguous
+        swap = [r[:] for r in lr]
+        swap[1], swap[2] = [swap[1][0], swap[2][1]], [swap[1][0] + swap[2][1], swap[1][1]]
+        short = lr[:-1]  # leaves the deepest level uncovered
+        for bad_lr in (past, gap, swap, short):
+            with pytest.raises(galaxy_hip.GalaxyError):
+                a.upload_tree(ref["nodes"], nn, bad_lr)
+        with pytest.raises(ValueError):
+            a.upload_tree(ref["nodes"][:-16], nn, lr)  # shorter than 16 x num_nodes
+        # the handle still holds the good tree
+        a.upload_tree(ref["nodes"], nn, lr)
+        a.force()
+        assert np.array_equal(a.accel()[0].view(np.uint32), acc_a.view(np.uint32))
     finally:
         a.destroy()
         b.destroy()
