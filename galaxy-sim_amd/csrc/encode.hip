@@ -279,9 +279,13 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
   flush_hist<BITS>(hist, sc);
 }
 
-// digit histograms of host-supplied keys in slot 0 (gs_radix_sort_indices)
+// digit histograms of caller-supplied keys in slot 0 (gs_radix_sort_indices
+// and its allocation-free form): keys[i] = src[i] below n, padding keys
+// 0xffffffff up to the end of the last active chunk (inside the capacity),
+// vals[i] = i.  src may equal keys.
 template <int BITS>
-__global__ __launch_bounds__(256) void hist_kernel(uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void hist_kernel(const uint32_t* src,
+                                                   uint32_t* keys,
                                                    uint32_t* __restrict__ vals,
                                                    const DevSlot* __restrict__ slots,
                                                    DevScalars* scs) {
@@ -292,8 +296,8 @@ __global__ __launch_bounds__(256) void hist_kernel(uint32_t* __restrict__ keys,
   __syncthreads();
   const uint32_t end = min(((sl.n + GS_CHUNK - 1) / GS_CHUNK) * GS_CHUNK, sl.cap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
-    if (i >= sl.n) keys[sl.poff + i] = 0xffffffffu;
-    const uint32_t code = keys[sl.poff + i];
+    const uint32_t code = i < sl.n ? src[sl.poff + i] : 0xffffffffu;
+    keys[sl.poff + i] = code;
     vals[sl.poff + i] = i;
     count_digits<BITS>(hist, code);
   }
@@ -352,13 +356,24 @@ hipError_t gs_launch_encode(gs_sim* s, const float4* posm, bool keep_raw) {
   return hipGetLastError();
 }
 
-hipError_t gs_launch_hist(gs_sim* s) {
-  const uint32_t g = grid_for(s->hs[0].d.cap, 256, 4096);
-  if (s->radix_bits == 10)
-    hist_kernel<10><<<g, 256, 0, s->stream>>>(s->keys[0], s->vals[0], s->slots, s->sc);
+hipError_t gs_launch_hist_bufs(const SortBufs& b, const uint32_t* src, uint32_t cap) {
+  const uint32_t g = grid_for(cap, 256, 4096);
+  if (b.radix_bits == 10)
+    hist_kernel<10><<<g, 256, 0, b.stream>>>(src, b.keys[0], b.vals[0], b.slots, b.sc);
   else
-    hist_kernel<8><<<g, 256, 0, s->stream>>>(s->keys[0], s->vals[0], s->slots, s->sc);
+    hist_kernel<8><<<g, 256, 0, b.stream>>>(src, b.keys[0], b.vals[0], b.slots, b.sc);
   return hipGetLastError();
+}
+
+hipError_t gs_launch_hist(gs_sim* s) {
+  SortBufs b;
+  b.keys[0] = s->keys[0];
+  b.vals[0] = s->vals[0];
+  b.slots = s->slots;
+  b.sc = s->sc;
+  b.radix_bits = s->radix_bits;
+  b.stream = s->stream;
+  return gs_launch_hist_bufs(b, s->keys[0], s->hs[0].d.cap);
 }
 
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st) {

@@ -306,13 +306,32 @@ inline uint64_t gs_sort_status_words(const gs_sim* s) {
   return (uint64_t)gs_sort_passes((int)s->radix_bits) * s->sort_tiles * (1u << s->radix_bits);
 }
 
+// Device buffers of one radix sort without the particle gather: a handle's
+// own, or the static scratch of the allocation-free entry point.
+struct SortBufs {
+  uint32_t* keys[2];           // pass p reads keys[p & 1], writes the other
+  uint32_t* vals[2];
+  uint32_t* status;            // passes x tiles x radix look-back words
+  DevBatch* batch;             // tile tickets
+  const DevSlot* slots;
+  DevScalars* sc;              // digit histograms (accumulated by the hist pass)
+  const uint32_t* tile_tab;
+  uint32_t tiles;              // grid of each pass
+  uint32_t radix_bits;         // 8 or 10
+  hipStream_t stream;
+};
+
 // ---- launchers (stream-ordered; return hipError_t of the launch) ----------
 hipError_t gs_launch_bbox(gs_sim* s, const float4* posm);
 hipError_t gs_launch_root(gs_sim* s);
 hipError_t gs_launch_encode(gs_sim* s, const float4* posm, bool keep_raw);
 hipError_t gs_launch_reset(gs_sim* s);
 hipError_t gs_launch_hist(gs_sim* s);  // histograms of host-supplied keys (slot 0)
+// slot 0 of b: keys[0][i] = src[i], vals[0][i] = i and the digit histograms,
+// over the slot's capacity (src may be keys[0])
+hipError_t gs_launch_hist_bufs(const SortBufs& b, const uint32_t* src, uint32_t cap);
 hipError_t gs_launch_sort(gs_sim* s, bool gather);
+hipError_t gs_launch_sort_bufs(const SortBufs& b);  // every pass, no gather
 hipError_t gs_launch_build(gs_sim* s);
 hipError_t gs_launch_refresh(gs_sim* s, const float4* posm);
 hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count);

@@ -224,29 +224,54 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
   }
 }
 
+// every pass over the buffers of b; the last pass gathers the particle
+// records of s when s is given
 template <int BITS>
-hipError_t launch_sort(gs_sim* s, bool gather) {
+hipError_t launch_sort(const SortBufs& b, gs_sim* s) {
   constexpr int RADIX = 1 << BITS, PASSES = gs_sort_passes(BITS);
   for (int p = 0; p < PASSES; p++) {
-    uint32_t* status = s->sort_status + (size_t)p * s->sort_tiles * RADIX;
-    const int a = p & 1, b = a ^ 1;
-    if (gather && p == PASSES - 1)
-      onesweep_kernel<BITS, true><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
-          s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
-          s->tile_tab, s->posm[s->cur], s->velp[s->cur], s->posm[s->cur ^ 1],
+    uint32_t* status = b.status + (size_t)p * b.tiles * RADIX;
+    const int x = p & 1, y = x ^ 1;
+    if (s && p == PASSES - 1)
+      onesweep_kernel<BITS, true><<<b.tiles, GS_SORT_BLOCK, 0, b.stream>>>(
+          b.keys[x], b.vals[x], b.keys[y], b.vals[y], p, status, b.batch, b.slots, b.sc,
+          b.tile_tab, s->posm[s->cur], s->velp[s->cur], s->posm[s->cur ^ 1],
           s->velp[s->cur ^ 1], s->stel[s->stel_cur], s->stel[s->stel_cur ^ 1]);
     else
-      onesweep_kernel<BITS, false><<<s->sort_tiles, GS_SORT_BLOCK, 0, s->stream>>>(
-          s->keys[a], s->vals[a], s->keys[b], s->vals[b], p, status, s->batch, s->slots, s->sc,
-          s->tile_tab, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+      onesweep_kernel<BITS, false><<<b.tiles, GS_SORT_BLOCK, 0, b.stream>>>(
+          b.keys[x], b.vals[x], b.keys[y], b.vals[y], p, status, b.batch, b.slots, b.sc,
+          b.tile_tab, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
+SortBufs bufs_of(gs_sim* s) {
+  SortBufs b;
+  for (int k = 0; k < 2; k++) {
+    b.keys[k] = s->keys[k];
+    b.vals[k] = s->vals[k];
+  }
+  b.status = s->sort_status;
+  b.batch = s->batch;
+  b.slots = s->slots;
+  b.sc = s->sc;
+  b.tile_tab = s->tile_tab;
+  b.tiles = s->sort_tiles;
+  b.radix_bits = s->radix_bits;
+  b.stream = s->stream;
+  return b;
+}
+
 }  // namespace
 
 hipError_t gs_launch_sort(gs_sim* s, bool gather) {
-  return s->radix_bits == 10 ? launch_sort<10>(s, gather) : launch_sort<8>(s, gather);
+  const SortBufs b = bufs_of(s);
+  return s->radix_bits == 10 ? launch_sort<10>(b, gather ? s : nullptr)
+                             : launch_sort<8>(b, gather ? s : nullptr);
+}
+
+hipError_t gs_launch_sort_bufs(const SortBufs& b) {
+  return b.radix_bits == 10 ? launch_sort<10>(b, nullptr) : launch_sort<8>(b, nullptr);
 }

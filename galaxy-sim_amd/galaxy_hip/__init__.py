@@ -95,6 +95,7 @@ def lib():
         "gs_force": (ctypes.c_int, [_vp]),
         "gs_step": (ctypes.c_int, [_vp, _u32, _u32]),
         "gs_walk_step": (ctypes.c_int, [_vp]),
+        "gs_radix_sort_indices_noalloc": (ctypes.c_int, [_vp, _u32, _vp, _vp]),
         "gs_sync": (ctypes.c_int, [_vp]),
         "gs_apply_permutation": (ctypes.c_int, [_vp, _vp, _u64]),
         "gs_upload_tree": (ctypes.c_int, [_vp, _vp, _u64, _vp, _u32]),

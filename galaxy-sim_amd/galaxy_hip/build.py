@@ -22,7 +22,7 @@ ADDON = os.path.join(PKG, "node", "galaxy_hip.node")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["encode.hip", "sort.hip", "build.hip", "refresh.hip", "walk.hip", "records.hip",
-           "stellar.hip", "api.hip"]
+           "stellar.hip", "api.hip", "noalloc.hip"]
 
 # -ffp-contract=off: every a*b+c in the kernels rounds twice, as the reference
 # expressions do (the walk's opening test and the FP64 COM sums depend on it).

@@ -291,6 +291,25 @@ int gs_tree_build(gs_sim* sim, const uint32_t* sorted_codes, const float* pos3,
                   uint32_t* nodes, uint64_t cap, uint64_t* num_nodes,
                   uint32_t* level_ranges, uint32_t* num_levels);
 
+/* ---- allocation-free form -------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): radixSortIndices (mortonTree.js:68-84, the same stable
+ * permutation as gs_radix_sort_indices) on DEVICE arrays, with no handle and
+ * no allocation.  codes: n Morton codes (30-bit values), perm: n outputs
+ * (perm[new] = old), both device pointers on the current device; stream: a
+ * hipStream_t of the current device, or NULL for the null stream.  Work is
+ * queued on the stream (kernels only, no memset, no host synchronisation).
+ * Its scratch is a static device array of the library, about 13 MB for
+ * GS_NOALLOC_MAX_KEYS keys, held on each device from its first use until the
+ * process ends.  n > GS_NOALLOC_MAX_KEYS returns GS_E_INVALID before any work
+ * (n = 0 does nothing); a failed launch returns GS_E_DEVICE.  Every call on
+ * one device shares that device's scratch: calls that overlap on a device
+ * (other streams, other threads) race and must be serialised by the caller. */
+#define GS_NOALLOC_MAX_KEYS (1u << 20)
+int gs_radix_sort_indices_noalloc(const uint32_t* codes, uint32_t n, uint32_t* perm,
+                                  void* stream);
+
 #ifdef __cplusplus
 }
 #endif
