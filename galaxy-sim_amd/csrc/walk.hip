@@ -96,6 +96,111 @@ __device__ __forceinline__ void pair_fast(float4 q, float4 p, float eps2, float&
   interact_fast(q.w, qx, qy, qz, dj2, ax, ay, az, pot);
 }
 
+// ---- transposed leaves (default path) --------------------------------------
+// A leaf opened by few lanes wastes most of the broadcast member loop (one
+// member per iteration, lanes that did not open the leaf idle).  Instead the
+// wavefront takes the (opening lane, member) pairs themselves: with the member
+// count padded to cp = 2^LCP <= 32, lane L holds member j = L mod cp of the
+// leaf (padding members weigh 0) and, per round, the particle of opening lane
+// number base + L / cp (positions staged in LDS by rank); each segment of cp
+// lanes sums its pairs with row-shift DPP adds into its last lane, which
+// stores the sum in LDS by rank; after the rounds every opening lane adds its
+// own sum.  64 / cp opening lanes per round.  The summation order differs
+// from the reference's, so only the default (tolerance) path uses it;
+// FASTLEAF semantics (the j == i pair adds r = 0) hold.
+struct WaveLds {
+  float4 pos[64];  // positions of the opening lanes, by rank
+  float4 sum[64];  // their sums (ax, ay, az, pot), by rank
+};
+
+template <int SH>
+__device__ __forceinline__ float row_shr_add(float v) {
+  // v + the value of the lane SH below in the same 16-lane row (0 past the
+  // row's first lane)
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 | SH, 0xf,
+                                                        0xf, true));
+}
+
+__device__ __forceinline__ float row_bcast15_add(float v) {
+  // rows 1 and 3 add the last lane of the row below (rows 0 and 2 add 0)
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xa, 0xf,
+                                                        false));
+}
+
+// inclusive sums over segments of 2^LCP lanes: each segment's last lane ends
+// with the sum of its segment
+template <int LCP>
+__device__ __forceinline__ void segment_sums(float& a, float& b, float& c, float& d) {
+  if (LCP > 0) {
+    a = row_shr_add<1>(a); b = row_shr_add<1>(b); c = row_shr_add<1>(c); d = row_shr_add<1>(d);
+  }
+  if (LCP > 1) {
+    a = row_shr_add<2>(a); b = row_shr_add<2>(b); c = row_shr_add<2>(c); d = row_shr_add<2>(d);
+  }
+  if (LCP > 2) {
+    a = row_shr_add<4>(a); b = row_shr_add<4>(b); c = row_shr_add<4>(c); d = row_shr_add<4>(d);
+  }
+  if (LCP > 3) {
+    a = row_shr_add<8>(a); b = row_shr_add<8>(b); c = row_shr_add<8>(c); d = row_shr_add<8>(d);
+  }
+  if (LCP > 4) {
+    a = row_bcast15_add(a); b = row_bcast15_add(b); c = row_bcast15_add(c);
+    d = row_bcast15_add(d);
+  }
+}
+
+// the transposed form pays off when its rounds cost less than the broadcast
+// loop's count iterations (per-round and per-leaf costs in vector
+// instructions, measured on the headline workload)
+#define GS_TR_ROUND 20u
+#define GS_TR_LCP 4u
+#define GS_TR_LEAF 12u
+__device__ __forceinline__ bool leaf_transposes(uint64_t open, uint32_t count, uint32_t& lcp) {
+  if (count < 2 || count > 32) return false;
+  lcp = 32u - (uint32_t)__builtin_clz(count - 1u);  // ceil(log2(count)), 1..5
+  const uint32_t per = 64u >> lcp;                  // opening lanes per round
+  const uint32_t k = (uint32_t)__builtin_popcountll(open);
+  const uint32_t rounds = (k + per - 1u) / per;
+  return rounds * (GS_TR_ROUND + GS_TR_LCP * lcp) + GS_TR_LEAF < 14u * count;
+}
+
+// the pairs of one leaf [first, first + count) for the opening lanes `open`
+// (every lane of the wavefront executes this; 2 <= count <= 2^LCP <= 32)
+template <int LCP>
+__device__ __forceinline__ void leaf_transposed(const float4* __restrict__ posm, uint32_t first,
+                                                uint32_t count, uint64_t open, uint32_t lane,
+                                                float4 p, float eps2, WaveLds* L, float& ax,
+                                                float& ay, float& az, float& pot) {
+  constexpr uint32_t CP = 1u << LCP, PER = 64u >> LCP;
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(open >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)open, 0u));
+  const bool mine = __builtin_amdgcn_inverse_ballot_w64(open);
+  if (mine) L->pos[rank] = p;
+  const uint32_t j = lane & (CP - 1u), g = lane >> LCP;
+  float4 q = posm[first + (j < count ? j : count - 1u)];
+  if (j >= count) q.w = 0.f;  // padding member: no force, no potential
+  const uint32_t k = (uint32_t)__builtin_popcountll(open);
+  __builtin_amdgcn_wave_barrier();
+  // base + g <= 63: base is a multiple of PER below k <= 64.  Segments past
+  // the last opening lane read stale positions; nobody reads their sums.
+  for (uint32_t base = 0; base < k; base += PER) {
+    const float4 pi = L->pos[base + g];
+    float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
+    pair_fast(q, pi, eps2, sx, sy, sz, sp);
+    segment_sums<LCP>(sx, sy, sz, sp);
+    if (j == CP - 1u) L->sum[base + g] = make_float4(sx, sy, sz, sp);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (mine) {
+    const float4 t = L->sum[rank];
+    ax += t.x;
+    ay += t.y;
+    az += t.z;
+    pot += t.w;
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -115,7 +220,7 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
                                               uint32_t i, bool valid, float G,
                                               float eps2, float theta2, uint32_t qbits,
                                               float& ax, float& ay, float& az, float& pot,
-                                              float& dens, uint32_t& cnt) {
+                                              float& dens, uint32_t& cnt, WaveLds* wl) {
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
   const float selfpot = FASTLEAF ? p.w * __builtin_amdgcn_rsqf(eps2) : 0.f;
   int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;  // node link (record byte offset)
@@ -223,7 +328,19 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
         dens = (float)count / (s2 * nsize[gs_node(nd)]);
         if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
       }
-      if (__builtin_amdgcn_inverse_ballot_w64(need_mask)) {  // lanes opening the leaf
+      uint32_t lcp = 0;
+      if (need_mask == 0) {
+        // no lane opens the leaf
+      } else if (FASTLEAF && leaf_transposes(need_mask, count, lcp)) {
+        const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+#define GS_TR(L) leaf_transposed<L>(posm, first, count, need_mask, lane, p, eps2, wl, ax, ay, az, pot)
+        if (lcp == 1) GS_TR(1);
+        else if (lcp == 2) GS_TR(2);
+        else if (lcp == 3) GS_TR(3);
+        else if (lcp == 4) GS_TR(4);
+        else GS_TR(5);
+#undef GS_TR
+      } else if (__builtin_amdgcn_inverse_ballot_w64(need_mask)) {  // lanes opening the leaf
         const uint32_t last = first + count;
         uint32_t j = first;
         if (FASTLEAF) {
@@ -290,6 +407,10 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     const uint32_t* __restrict__ massless, DevBatch* batch, double* __restrict__ red) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
+  // transposed-leaf staging of this wavefront (dynamic LDS sized to the
+  // block's wavefronts; only the default force path launches it)
+  extern __shared__ WaveLds s_leaf[];
+  WaveLds* wl = (EXACT || COUNT) ? nullptr : &s_leaf[w];
 
   // a workgroup never spans two slots (slot ranges are GS_SLOT_ALIGN-aligned).
   // The hardware deals workgroups to the 8 XCDs round-robin (XCD = id % 8);
@@ -346,7 +467,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   uint32_t cnt = 0;
 #define GS_WALK(FL, QC)                                                                      \
   walk_particle<EXACT, COUNT, FL, QC>(nodes, nsize, posm, p, i, valid, G, eps2, theta2,      \
-                                      qbits, ax, ay, az, pot, dens, cnt)
+                                      qbits, ax, ay, az, pot, dens, cnt, wl)
   if (fastleaf) {
     if (generic) GS_WALK(true, true);
     else GS_WALK(true, false);
@@ -462,7 +583,8 @@ __global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restr
 template <bool EXACT, bool STEP, bool COUNT>
 hipError_t launch(gs_sim* s) {
   const int c = s->cur;
-  walk_kernel<EXACT, STEP, COUNT><<<s->walk_blocks, s->walk_wg, 0, s->stream>>>(
+  const size_t lds = (EXACT || COUNT) ? 0 : (size_t)(s->walk_wg / 64) * sizeof(WaveLds);
+  walk_kernel<EXACT, STEP, COUNT><<<s->walk_blocks, s->walk_wg, lds, s->stream>>>(
       s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
       s->inter, s->nodes, s->nsize, s->slots, s->wblock_slot, s->prm, s->sc, s->massless,
       s->batch, s->red_part);
