@@ -209,6 +209,7 @@ void pack_params(const gs_params* p, uint32_t n, DevParams* d) {
                            d->halo_count > 1 ? (float)p->halo_centers[1][2] : 0.f, 0.f);
   d->theta2 = d->theta * d->theta;
   d->drag_damp = expf(-d->gas_drag * d->dt);
+  d->qsize_bits = gs_quarter_eps_size(0.25f * d->eps2);
 }
 
 void push_slot(gs_sim* s, uint32_t slot) {
@@ -703,6 +704,8 @@ int gs_slot_set_params(gs_sim* s, uint32_t slot, const gs_params* p) {
     need(p != nullptr, GS_E_INVALID, "null params");
     pack_params(p, h.d.n, &s->host_prm[slot]);
     push_params(s, slot);
+    // the tree's records hold opening thresholds for the old theta
+    if (h.have_tree) check(gs_launch_rethreshold(s, slot), "rethreshold");
     sync(s);
     h.have_params = true;
   });
@@ -1223,6 +1226,8 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
       sz *= 0.5;
     }
     sc.min_s2 = min_s2;
+    for (int d = 0; d <= GS_MAX_LEVELS; d++)
+      sc.level_thr[d] = gs_open_threshold(host_s2(sc.level_size[d]), s->host_prm[0].theta2);
     memcpy(&sc.size_bits, &root_size, 8);
     for (int d = 0; d <= GS_MAX_LEVELS; d++) sc.level_start[d] = sc.level_count[d] = 0;
     sc.level_count[0] = 1;
@@ -1233,7 +1238,7 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     memset(&b, 0, sizeof b);
     check(hipMemcpyAsync(s->batch, &b, sizeof b, hipMemcpyHostToDevice, s->stream), "batch");
     const float f0 = sc.level_size[0];
-    const int4 root = make_int4(gs_s2_bits(f0), 0, (int)gs_meta_open(0, n), -1);
+    const int4 root = make_int4((int)sc.level_thr[0], 0, (int)gs_meta_open(0, n), -1);
     check(hipMemcpyAsync(&s->nodes[h.d.noff].info, &root, sizeof root, hipMemcpyHostToDevice,
                          s->stream),
           "root");

@@ -96,10 +96,11 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const uint32_t count = sc->level_count[d];
     const uint32_t start = sc->level_start[d];
     const uint32_t next_start = start + count;
-    // records hold size * size; the children's sizes also go to nsize
-    const int s2_d = gs_s2_bits(sc->level_size[d]);
+    // records hold the opening threshold of their depth's size (the root
+    // kernel's table for the slot's theta^2); the sizes go to nsize
+    const int s2_d = (int)sc->level_thr[d];
     const float size_c = sc->level_size[d + 1];
-    const int s2_c = gs_s2_bits(size_c);
+    const int s2_c = (int)sc->level_thr[d + 1];
 
     // phase 1: octant boundaries b_0..b_7 (+ hi), 8 lanes per node; each
     // thread owns one octant of R nodes and advances their R binary searches
@@ -378,13 +379,28 @@ __global__ __launch_bounds__(256) void node64_kernel(const NodeRec* __restrict__
   }
 }
 
+// ---- opening thresholds after a change of theta (updateParams) -----------
+// Every record of the slot's tree gets the threshold of its own size for the
+// new theta^2 (trees imported from the host may hold any sizes), and the
+// depth table follows for the next build.
+__global__ __launch_bounds__(256) void rethreshold_kernel(NodeRec* __restrict__ nodes,
+                                                          const float* __restrict__ nsize,
+                                                          DevScalars* sc, float theta2) {
+  const uint32_t num = sc->num_nodes;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g <= GS_MAX_LEVELS)
+    sc->level_thr[g] = gs_open_threshold(__int_as_float(gs_s2_bits(sc->level_size[g])), theta2);
+  for (uint32_t i = g; i < num; i += gridDim.x * blockDim.x)
+    nodes[i].info.x = (int)gs_open_threshold(__int_as_float(gs_s2_bits(nsize[i])), theta2);
+}
+
 // ---- import of a tree in the reference's 64-byte layout (uploadBHTree) ----
 __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nodes,
                                                        float* __restrict__ nsize,
                                                        DevBatch* batch,
                                                        const uint32_t* __restrict__ in,
                                                        const int2* __restrict__ links,
-                                                       uint32_t num) {
+                                                       uint32_t num, float theta2) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < num; i += gridDim.x * blockDim.x) {
     const uint4* r = (const uint4*)(in + (size_t)i * 16);
     const uint4 a = r[0], b = r[1], c = r[2], d = r[3];
@@ -393,7 +409,7 @@ __global__ __launch_bounds__(256) void import64_kernel(NodeRec* __restrict__ nod
     const int32_t skip = links[i].x < 0 ? -1 : gs_link((uint32_t)links[i].x);
     const uint32_t depth = (uint32_t)links[i].y;
     const float size = __uint_as_float(b.x);
-    const int s2 = gs_s2_bits(size);
+    const int s2 = (int)gs_open_threshold(__int_as_float(gs_s2_bits(size)), theta2);
     nsize[i] = size;
     if (b.y == 1u) {
       nodes[i].info = make_int4(s2, (int)b.z, (int)gs_meta_leaf(depth, b.w), skip);
@@ -453,6 +469,15 @@ hipError_t gs_launch_node64(gs_sim* s, uint32_t slot) {
   return hipGetLastError();
 }
 
+hipError_t gs_launch_rethreshold(gs_sim* s, uint32_t slot) {
+  const HostSlot& h = s->hs[slot];
+  uint32_t grid = (h.d.ncap + 255) / 256;
+  if (grid > 2048) grid = 2048;
+  rethreshold_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + h.d.noff, s->nsize + h.d.noff,
+                                                  s->sc + slot, s->host_prm[slot].theta2);
+  return hipGetLastError();
+}
+
 hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, const int2* links,
                               uint32_t num) {
   uint32_t grid = (num + 255) / 256;
@@ -460,7 +485,7 @@ hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, c
   if (grid < 1) grid = 1;
   import64_kernel<<<grid, 256, 0, s->stream>>>(s->nodes + s->hs[slot].d.noff,
                                                 s->nsize + s->hs[slot].d.noff, s->batch, words,
-                                                links, num);
+                                                links, num, s->host_prm[slot].theta2);
   return hipGetLastError();
 }
 

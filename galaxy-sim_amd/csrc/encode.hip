@@ -118,6 +118,7 @@ __global__ __launch_bounds__(256) void bbox_kernel(const float4* __restrict__ po
 
 __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ part,
                                                    const DevSlot* __restrict__ slots,
+                                                   const DevParams* __restrict__ prm,
                                                    DevScalars* scs, NodeRec* nodes,
                                                    float* nsize) {
   const uint32_t slot = blockIdx.x;
@@ -143,68 +144,76 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
     s[w][6] = __uint_as_float(c);
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
-    for (int j = 0; j < 3; j++) r[j] = fminf(r[j], s[k][j]);
-    for (int j = 3; j < 6; j++) r[j] = fmaxf(r[j], s[k][j]);
-  }
-  c = 0;
-  for (int k = 0; k < (int)(blockDim.x >> 6); k++) c += __float_as_uint(s[k][6]);
-  sc->nfinite = c;
-  uint64_t size;
-  if (c == 0) {
-    // no finite particle: minX = +Inf, size = -Inf, s = -0 -> every code is 0
-    size = 0xfff0000000000000ull;
-    for (int j = 0; j < 3; j++) {
-      sc->min_bits[j] = 0x7ff0000000000000ull;
-      sc->min_f32[j] = INFINITY;
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+      for (int j = 0; j < 3; j++) r[j] = fminf(r[j], s[k][j]);
+      for (int j = 3; j < 6; j++) r[j] = fmaxf(r[j], s[k][j]);
     }
-    sc->s_bits = 0x8000000000000000ull;
-    sc->s32 = -0.0f;
-  } else {
-    // the JS Number arithmetic in integer soft-double, each operation
-    // correctly rounded: d = max - min per axis (>= +0), size =
-    // max(d) * 1.0001 + 1e-9, s = 1024 / size
-    uint64_t d[3];
-    for (int j = 0; j < 3; j++) {
-      const uint64_t mn = sd::from_f32(__float_as_uint(r[j]));
-      sc->min_bits[j] = mn;
-      sc->min_f32[j] = r[j];
-      d[j] = sd::sub(sd::from_f32(__float_as_uint(r[j + 3])), mn);
+    c = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) c += __float_as_uint(s[k][6]);
+    sc->nfinite = c;
+    uint64_t size;
+    if (c == 0) {
+      // no finite particle: minX = +Inf, size = -Inf, s = -0 -> every code is 0
+      size = 0xfff0000000000000ull;
+      for (int j = 0; j < 3; j++) {
+        sc->min_bits[j] = 0x7ff0000000000000ull;
+        sc->min_f32[j] = INFINITY;
+      }
+      sc->s_bits = 0x8000000000000000ull;
+      sc->s32 = -0.0f;
+    } else {
+      // the JS Number arithmetic in integer soft-double, each operation
+      // correctly rounded: d = max - min per axis (>= +0), size =
+      // max(d) * 1.0001 + 1e-9, s = 1024 / size
+      uint64_t d[3];
+      for (int j = 0; j < 3; j++) {
+        const uint64_t mn = sd::from_f32(__float_as_uint(r[j]));
+        sc->min_bits[j] = mn;
+        sc->min_f32[j] = r[j];
+        d[j] = sd::sub(sd::from_f32(__float_as_uint(r[j + 3])), mn);
+      }
+      uint64_t m = d[0];
+      if (sd::lt(m, d[1])) m = d[1];
+      if (sd::lt(m, d[2])) m = d[2];
+      const uint64_t k10001 = 0x3FF00068DB8BAC71ull;  // 1.0001
+      const uint64_t k1em9 = 0x3E112E0BE826D695ull;   // 1e-9
+      size = sd::add(sd::mul(m, k10001), k1em9);
+      sc->s_bits = sd::div(0x4090000000000000ull /* 1024.0 */, size);
+      sc->s32 = __uint_as_float(sd::to_f32(sc->s_bits));
     }
-    uint64_t m = d[0];
-    if (sd::lt(m, d[1])) m = d[1];
-    if (sd::lt(m, d[2])) m = d[2];
-    const uint64_t k10001 = 0x3FF00068DB8BAC71ull;  // 1.0001
-    const uint64_t k1em9 = 0x3E112E0BE826D695ull;   // 1e-9
-    size = sd::add(sd::mul(m, k10001), k1em9);
-    sc->s_bits = sd::div(0x4090000000000000ull /* 1024.0 */, size);
-    sc->s32 = __uint_as_float(sd::to_f32(sc->s_bits));
+    sc->size_bits = size;
+    // node sizes per depth: f32(rootSize * 0.5^d) (mortonTree.js:124,161)
+    uint64_t sz = size;
+    float min_s2 = INFINITY;
+    for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
+      const float f = __uint_as_float(sd::to_f32(sz));
+      sc->level_size[dpt] = f;
+      // nodes exist at depths 0..GS_MAX_LEVEL only; fminf drops a NaN
+      if (dpt <= GS_MAX_LEVEL) min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));
+      if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
+    }
+    sc->min_s2 = min_s2;
+    // level 0 holds the root alone (none for an empty slot)
+    for (int k = 0; k <= GS_MAX_LEVELS; k++) {
+      sc->level_start[k] = 0;
+      sc->level_count[k] = 0;
+    }
+    sc->level_count[0] = sl.n ? 1u : 0u;
+    sc->num_nodes = sl.n ? 1u : 0u;
+    sc->num_levels = 0;
   }
-  sc->size_bits = size;
-  // node sizes per depth: f32(rootSize * 0.5^d) (mortonTree.js:124,161)
-  uint64_t sz = size;
-  float min_s2 = INFINITY;
-  for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
-    const float f = __uint_as_float(sd::to_f32(sz));
-    sc->level_size[dpt] = f;
-    // nodes exist at depths 0..GS_MAX_LEVEL only; fminf drops a NaN
-    if (dpt <= GS_MAX_LEVEL) min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));
-    if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
-  }
-  sc->min_s2 = min_s2;
+  __syncthreads();
+  // opening thresholds of each depth's size for the slot's theta^2, one lane
+  // per depth (their bisections run side by side)
+  if (threadIdx.x <= GS_MAX_LEVELS)
+    sc->level_thr[threadIdx.x] =
+        gs_open_threshold(__int_as_float(gs_s2_bits(sc->level_size[threadIdx.x])), prm[slot].theta2);
+  __syncthreads();
   // level 0 = the root, covering [0, n); an empty slot has no tree
-  for (int k = 0; k <= GS_MAX_LEVELS; k++) {
-    sc->level_start[k] = 0;
-    sc->level_count[k] = 0;
-  }
-  sc->level_count[0] = sl.n ? 1u : 0u;
-  sc->num_nodes = sl.n ? 1u : 0u;
-  sc->num_levels = 0;
-  if (sl.n) {
+  if (threadIdx.x == 0 && sl.n) {
     nsize[sl.noff] = sc->level_size[0];
-    nodes[sl.noff].info = make_int4(gs_s2_bits(sc->level_size[0]), 0,
-                                    (int)gs_meta_open(0, sl.n), -1);
+    nodes[sl.noff].info = make_int4((int)sc->level_thr[0], 0, (int)gs_meta_open(0, sl.n), -1);
   }
 }
 
@@ -343,8 +352,8 @@ hipError_t gs_launch_bbox(gs_sim* s, const float4* posm) {
 }
 
 hipError_t gs_launch_root(gs_sim* s) {
-  root_kernel<<<s->nslots, 256, 0, s->stream>>>(s->bbox_part, s->slots, s->sc, s->nodes,
-                                                 s->nsize);
+  root_kernel<<<s->nslots, 256, 0, s->stream>>>(s->bbox_part, s->slots, s->prm, s->sc,
+                                                 s->nodes, s->nsize);
   return hipGetLastError();
 }
 

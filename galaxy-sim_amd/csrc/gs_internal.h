@@ -17,7 +17,10 @@
 //   keys[2][P], vals[2][P]   u32         radix-sort ping-pong (code, index)
 //   node[C]     32-byte record (one s_load_dwordx8 in the walk):
 //               float4 cm   (comx, comy, comz, mass)
-//               int4   info (size*size bits, first | lastChild, meta, skip)
+//               int4   info (opening threshold bits, first | lastChild, meta,
+//                            skip); the threshold T of the node's size for
+//                            the slot's theta^2: accept <=> d^2 > T
+//                            (gs_open_threshold)
 //               (lastChild and skip are byte offsets of 32-byte records,
 //               i.e. 32 x the node index, so the walk loads a record without
 //               an index shift; -1 ends the walk)
@@ -27,9 +30,9 @@
 //               skip: the node that follows this node's subtree in the
 //                     walk's pre-order (previous sibling, else the parent's
 //                     skip; -1 ends the walk)
-//   nsize[C]    float node size (the record holds its square, which is what the
-//               walk's opening test reads; the size itself is read only for
-//               the density of a particle's own leaf and for downloads)
+//   nsize[C]    float node size (read for the density of a particle's own
+//               leaf, the s^2 < eps^2/4 test of trees that need it, downloads
+//               and threshold rewrites)
 // Node and particle indices inside a slot are slot-local (0-based).  A slot's
 // node pool holds min(2 cap + 4096, 2^27) nodes (mortonTree.js:88).  Children
 // of a BFS node are contiguous, so an internal node stores (lastChild, nChild,
@@ -69,6 +72,10 @@ struct DevParams {
   uint32_t halo_count;
   float theta2;     // f32(theta * theta) as the WGSL evaluates it
   float drag_damp;  // f32 exp(-gasDrag * dt), evaluated once on the host
+  // bits of the smallest size with f32(size * size) >= f32(0.25 * eps2): a
+  // node is accepted by s^2 < eps^2/4 iff |size| bits are below this
+  uint32_t qsize_bits;
+  uint32_t pad_[3];
   float4 halo_c1, halo_c2;
 };
 
@@ -114,6 +121,9 @@ struct DevScalars {
   uint32_t num_nodes;
   uint32_t num_levels;
   float min_s2;          // smallest size * size over the tree's nodes (NaN ignored)
+  // opening thresholds of each depth's node size for the slot's theta^2
+  // (gs_open_threshold), written by the root kernel for the build
+  uint32_t level_thr[GS_MAX_LEVELS + 1];
   uint32_t level_start[GS_MAX_LEVELS + 1];
   uint32_t level_count[GS_MAX_LEVELS + 1];
   float level_size[GS_MAX_LEVELS + 1];
@@ -137,8 +147,8 @@ struct __align__(32) NodeRec {
   int4 info;
 };
 
-// the record's first info word: f32(size * size) as the walk's opening test
-// evaluates it (nbody_bh.wgsl:99), one rounding
+// f32(size * size) as the walk's opening test evaluates it
+// (nbody_bh.wgsl:99), one rounding, as bits
 __host__ __device__ __forceinline__ int gs_s2_bits(float size) {
   const float s2 = size * size;
   union {
@@ -147,6 +157,49 @@ __host__ __device__ __forceinline__ int gs_s2_bits(float size) {
   } u;
   u.f = s2;
   return u.i;
+}
+
+// The walk's opening test s2 < f32(theta2 * d2) (nbody_bh.wgsl:97-112) as one
+// compare d2 > T per node.  accept(x) = s2 < f32(theta2 * x) is monotone in x
+// (rounding is monotone; theta2 >= 0 or NaN, s2 >= 0 or NaN), and false at
+// x = +0, so T = max{x in [+0, +inf] : !accept(x)} (bits returned) gives
+// accept(x) <=> x > T for every x >= 0, and false for NaN on both sides.
+// Non-negative floats order as their bit patterns: bisection over the bits.
+__host__ __device__ inline bool gs_accepts(float s2, float theta2, uint32_t xbits) {
+  union {
+    uint32_t u;
+    float f;
+  } x;
+  x.u = xbits;
+  return s2 < theta2 * x.f;
+}
+__host__ __device__ inline uint32_t gs_open_threshold(float s2, float theta2) {
+  const uint32_t inf = 0x7f800000u;
+  if (!gs_accepts(s2, theta2, inf)) return inf;  // never accepted
+  uint32_t lo = 0u, hi = inf;                    // !accept(lo), accept(hi)
+  while (hi - lo > 1u) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (gs_accepts(s2, theta2, mid)) hi = mid;
+    else lo = mid;
+  }
+  return lo;
+}
+// smallest size bits with f32(size * size) >= q: a node passes s^2 < q
+// (q = eps^2/4) iff its |size| bits are below it (NaN sizes never pass)
+__host__ __device__ inline uint32_t gs_quarter_eps_size(float q) {
+  if (!(q > 0.0f)) return 0u;  // q NaN or <= 0: no size passes
+  union {
+    uint32_t u;
+    float f;
+  } x;
+  uint32_t lo = 0u, hi = 0x7f800000u;  // (lo * lo) < q, (hi * hi) >= q
+  while (hi - lo > 1u) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    x.u = mid;
+    if (x.f * x.f >= q) hi = mid;
+    else lo = mid;
+  }
+  return hi;
 }
 
 // node meta word
@@ -339,6 +392,8 @@ hipError_t gs_launch_reduce_final(gs_sim* s, uint32_t slot, double* out_dev);
 hipError_t gs_launch_node64(gs_sim* s, uint32_t slot);
 hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, const int2* links,
                               uint32_t num);
+// rewrite the opening thresholds of one slot's tree for its current theta^2
+hipError_t gs_launch_rethreshold(gs_sim* s, uint32_t slot);
 hipError_t gs_launch_gather(gs_sim* s, uint32_t slot, const uint32_t* perm);
 hipError_t gs_launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t gs_launch_star_formation(gs_sim* s);  // every slot, in place

@@ -218,7 +218,7 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
                                               const float* __restrict__ nsize,
                                               const float4* __restrict__ posm, float4 p,
                                               uint32_t i, bool valid, float G,
-                                              float eps2, float theta2, uint32_t qbits,
+                                              float eps2, uint32_t qsize,
                                               float& ax, float& ay, float& az, float& pot,
                                               float& dens, uint32_t& cnt, WaveLds* wl) {
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
@@ -258,17 +258,17 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       asm("s_lshl_b32 %0, %1, 1" : "=s"(mbits) : "s"(__float_as_uint(c.w)));
       dtest = mbits != 0u ? (int32_t)depth : 64;
     }
-    const float s2 = __int_as_float(inf.x);  // the record holds size * size
+    const float thr = __int_as_float(inf.x);  // opening threshold: accept iff d2 > thr
     const bool leaf = gs_meta_is_leaf(meta);
     const uint32_t first = (uint32_t)inf.y;  // leaf: first member; internal: last child
     const float rx = c.x - p.x, ry = c.y - p.y, rz = c.z - p.z;
     const float d2 = ((rx * rx + ry * ry) + rz * rz) + eps2;  // exact: drives the decision
     // monopole iff s2 < theta2*d2 || s2 < eps2/4 (nbody_bh.wgsl:97-112).  The
-    // first test is one compare per lane (false for a NaN d2, as in the
-    // reference); the second depends on the node alone and is an integer
-    // compare of non-negative float bits on the scalar unit, made only when
-    // some node of the tree is that small (GENERIC).  Every other active lane
-    // opens the node.
+    // first test is one compare per lane, d2 > thr, with the record's exact
+    // threshold of s2 and theta2 (false for a NaN d2, as in the reference);
+    // the second depends on the node alone and is an integer compare of
+    // float bits on the scalar unit, made only when some node of the tree is
+    // that small (GENERIC).  Every other active lane opens the node.
     uint64_t act, need_mask;
     if (!EXACT && !COUNT && !GENERIC) {
       // the two compares write the execution mask themselves (v_cmpx writes
@@ -277,12 +277,11 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       // (interact_fast, same instructions and order) runs under that mask;
       // EXEC is restored before the block ends.  Saves the scalar AND and
       // save-exec of the compiled branch on every visit.
-      const float t = theta2 * d2;
       uint64_t mono;
       float inv, mi;
       asm volatile(
           "v_cmpx_le_i32_e64 %[act], %[dep], %[alim]\n\t"
-          "v_cmpx_lt_f32_e64 %[mono], %[s2], %[t]\n\t"
+          "v_cmpx_lt_f32_e64 %[mono], %[thr], %[d2]\n\t"
           "s_cbranch_execz 1f\n\t"
           "v_rsq_f32_e32 %[inv], %[d2]\n\t"
           "s_nop 0\n\t"
@@ -297,15 +296,16 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
           "s_mov_b64 exec, %[full]"
           : [act] "=&s"(act), [mono] "=&s"(mono), [inv] "=&v"(inv), [mi] "=&v"(mi),
             [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [pot] "+v"(pot)
-          : [dep] "s"(dtest), [alim] "v"(alim), [s2] "s"(s2), [t] "v"(t), [d2] "v"(d2),
+          : [dep] "s"(dtest), [alim] "v"(alim), [thr] "s"(thr), [d2] "v"(d2),
             [m] "s"(c.w), [rx] "v"(rx), [ry] "v"(ry), [rz] "v"(rz), [full] "s"(full)
           : "scc");
       need_mask = act & ~mono;
     } else {
       // the lanes that opened every ancestor, as a wave mask (one compare)
       act = __builtin_amdgcn_ballot_w64(dtest <= alim);
-      uint64_t mono_raw = __builtin_amdgcn_ballot_w64(s2 < theta2 * d2);
-      if (GENERIC && (uint32_t)inf.x < qbits) mono_raw = ~0ull;
+      uint64_t mono_raw = __builtin_amdgcn_ballot_w64(thr < d2);
+      if (GENERIC && (__float_as_uint(nsize[gs_node(nd)]) & 0x7fffffffu) < qsize)
+        mono_raw = ~0ull;
       const uint64_t mono_mask = act & mono_raw;
       need_mask = act & ~mono_raw;
       if (__builtin_amdgcn_inverse_ballot_w64(mono_mask)) {
@@ -325,7 +325,8 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       // lane that opens its own leaf takes back the self potential that the
       // member loop below adds.
       if (__builtin_amdgcn_inverse_ballot_w64(act) && i - first < count) {
-        dens = (float)count / (s2 * nsize[gs_node(nd)]);
+        const float sz = nsize[gs_node(nd)];
+        dens = (float)count / ((sz * sz) * sz);
         if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
       }
       uint32_t lcp = 0;
@@ -439,10 +440,10 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     return;
   }
   const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float G = prm->G, eps2 = prm->eps2, theta2 = prm->theta2;
-  // eps2/4 as float bits for the scalar unit's compare (none when NaN)
+  const float G = prm->G, eps2 = prm->eps2;
   const float qeps2 = 0.25f * eps2;
-  const uint32_t qbits = qeps2 == qeps2 ? __float_as_uint(qeps2) : 0u;
+  // the node sizes that pass s^2 < eps^2/4 are those below qsize (as bits)
+  const uint32_t qsize = prm->qsize_bits;
   const float* __restrict__ nsize = nsize_all + sl.noff;
 
   // the eps^2/4 test can only accept a node if some node of the tree has
@@ -466,8 +467,8 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
   uint32_t cnt = 0;
 #define GS_WALK(FL, QC)                                                                      \
-  walk_particle<EXACT, COUNT, FL, QC>(nodes, nsize, posm, p, i, valid, G, eps2, theta2,      \
-                                      qbits, ax, ay, az, pot, dens, cnt, wl)
+  walk_particle<EXACT, COUNT, FL, QC>(nodes, nsize, posm, p, i, valid, G, eps2, qsize, ax,   \
+                                      ay, az, pot, dens, cnt, wl)
   if (fastleaf) {
     if (generic) GS_WALK(true, true);
     else GS_WALK(true, false);
