@@ -5,7 +5,8 @@ on one MI355X, headline workload = collision of two 2,097,152-particle disks
 step.  Prints ONE JSON line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
-                  [--no-batch] [--no-serve] [--only headline|batch|serve]
+                  [--no-configs] [--no-batch] [--no-serve]
+                  [--only headline|configs|batch|serve]
 
 A step is gs_step(sim, 1, 1): bbox + Morton encode, 4 radix passes (last one
 gathers the particle records), 11 BFS build levels, FP64 refresh, walk with
@@ -16,14 +17,23 @@ per-step latencies give p50/p99.  Each timed step replays two captured graphs
 own stream between them, which time the walk kernel over the timed region;
 a few more steps with direct launches give the per-stage breakdown.
 
-The same line carries two more objects (skipped with --no-batch/--no-serve):
+Beside the headline the line carries, per plain-form workload, the GPU rate
+with p50/p99, the reference step timed on the host cores through the oracle
+(cpu_baseline / "cpu": one warm-up step, then full steps of every particle),
+and the FP64 direct-sum force error on a seeded sample:
+  configs.cpu_disk_16k  N=16,384 disk, theta 0.7, eps 0.02, bucket 16;
+  configs.disk_500k     N=500,000 disk with 30 % gas, Hernquist halo (M 5,
+                        a 3) and drag 0.25, theta 0.7, eps 0.01, 100 steps;
+  configs.collision_4m  the headline.
+and two more objects (skipped with --no-batch/--no-serve):
   batched  64 galaxies of N log-uniform in [4096, 262144] stepped as one slot
            pool (one graph replay per step), walk workgroup / bucket / digit
            width from the autotuner, against the same galaxies stepped with
            per-galaxy launches;
   served   a Poisson job stream (20 jobs/s, N log-uniform in [8192, 2^20],
            200 steps each) served by continuous batching in real time, with
-           step p50/p99 and the one-job-at-a-time throughput beside it.
+           step p50/p99, queue wait, HBM budget use, the one-job-at-a-time
+           throughput and the FP64 force error of three of its jobs.
 """
 import argparse
 import json
@@ -42,33 +52,135 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(data, n, theta, eps, bucket, sample):
-    """The oracle (C restatement of mortonTree.js + FP32 transliteration of the
-    WGSL walk) on this host: sortAndBuild + FP64 refresh on all N particles,
-    single-threaded as the reference's worker; the walk over all host cores on
-    a seeded sample of particles, extrapolated to N."""
+def _halo_params(c):
+    """Halo / drag parameters of a workload: the 500,000-particle disk carries
+    a Hernquist halo (M 5, a 3) at the origin and gas drag 0.25."""
+    if "halo_mass" not in c:
+        return {}, ()
+    return (dict(haloMass=c["halo_mass"], haloA=c["halo_a"], gasDrag=c["gas_drag"]),
+            [(0.0, 0.0, 0.0)])
+
+
+def cpu_reference_steps(data, c, steps, warmup=1):
+    """The reference step on this host's cores through the oracle (C restatement
+    of mortonTree.js + FP32 transliteration of the WGSL walk): sortAndBuild and
+    the FP64 refresh on one thread, as the reference's worker runs them, then
+    the walk with halos, drag and kick-drift over every particle on all the
+    cores this process may use.  Each step starts from the previous one's
+    particles.  Returns per-step wall times (s) of the timed steps."""
     from oracle import oracle
-    cores = len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
-    ref = oracle.sort_and_build(data.reshape(-1), n, bucket=bucket)
-    t_build = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.refresh_f64(ref["sorted"], ref["nodes"])
-    t_refresh = time.perf_counter() - t0
-    rng = np.random.default_rng(12345)
-    idx = np.sort(rng.choice(n, sample, replace=False)).astype(np.uint32)
-    op = oracle.pack_params(n, dt=0.005, G=1.0, epsilon=eps, theta=theta)
-    t0 = time.perf_counter()
-    _, inter = oracle.walk_subset(ref["sorted"], ref["nodes"], op, idx, nthreads=cores)
-    t_walk = (time.perf_counter() - t0) * n / sample
-    step = t_build + t_refresh + t_walk
+    n = data.shape[0]
+    extra, halo = _halo_params(c)
+    op = oracle.pack_params(n, dt=c["dt"], G=1.0, epsilon=c["epsilon"], theta=c["theta"],
+                            halo_mass=extra.get("haloMass", 0.0), halo_a=extra.get("haloA", 1.0),
+                            gas_drag=extra.get("gasDrag", 0.0), halo_centers=halo)
+    threads = oracle.default_threads()
+    cur = data.reshape(-1).copy()
+    lat, parts = [], {"build": 0.0, "refresh": 0.0, "walk": 0.0}
+    for k in range(warmup + steps):
+        t0 = time.perf_counter()
+        ref = oracle.sort_and_build(cur, n, bucket=c["bucket"])
+        t1 = time.perf_counter()
+        nodes = oracle.refresh_f64(ref["sorted"], ref["nodes"])
+        t2 = time.perf_counter()
+        cur, _, _, _, _ = oracle.walk(ref["sorted"], nodes, op, nthreads=threads)
+        t3 = time.perf_counter()
+        if k >= warmup:
+            lat.append(t3 - t0)
+            parts["build"] += t1 - t0
+            parts["refresh"] += t2 - t1
+            parts["walk"] += t3 - t2
+    lat = np.array(lat)
     return {
-        "value": n / step, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-        "sample": (f"sortAndBuild + FP64 refresh of all {n} particles on 1 thread "
-                   f"({t_build:.2f} s + {t_refresh:.2f} s); walk of {sample} seeded-sampled "
-                   f"particles on {cores} threads extrapolated to N ({t_walk:.2f} s)"),
-        "step_s": step, "interactions_per_particle": float(inter.mean()),
+        "value": n * steps / lat.sum(), "unit": "particle-steps/s", "cores": threads,
+        "kind": "port", "steps": steps, "warmup": warmup,
+        "step_s": float(lat.mean()), "p50_s": float(np.percentile(lat, 50)),
+        "p99_s": float(np.percentile(lat, 99)),
+        "stages_s": {k: v / steps for k, v in parts.items()},
+        "sample": (f"{steps} full steps of all {n} particles after {warmup} warm-up: "
+                   f"sortAndBuild + FP64 refresh on 1 thread, walk + kick-drift on {threads} "
+                   f"threads"),
     }
+
+
+def force_error(acc, parts, eps, sample, seed):
+    """Relative error |a - a_ds| / |a_ds| of the GPU forces against the FP64
+    direct sum (oracle.direct_sum_f64) on a seeded sample of particles (all of
+    them when sample is None or at least N)."""
+    from oracle import oracle
+    n = parts.shape[0]
+    if sample is None or sample >= n:
+        idx = np.arange(n, dtype=np.uint32)
+    else:
+        idx = np.sort(np.random.default_rng(seed).choice(n, sample, replace=False)).astype(np.uint32)
+    ds = oracle.direct_sum_f64(parts.reshape(-1), 1.0, float(np.float32(eps * eps)), idx)
+    a = acc[idx, :3].astype(np.float64)
+    err = np.linalg.norm(a - ds, axis=1) / np.linalg.norm(ds, axis=1)
+    return {"particles": int(idx.size), "median": float(np.median(err)),
+            "p99": float(np.quantile(err, 0.99)), "max": float(err.max()),
+            "rms": float(np.sqrt(np.mean(err * err)))}
+
+
+def gpu_forces(data, bucket, theta, eps, exact=False):
+    """Default-path accelerations of one force evaluation (sort, refresh, walk)
+    and the sorted particles they belong to."""
+    import galaxy_hip
+    sim = galaxy_hip.NBodyGPU(data, bucket=bucket, exact_force=exact)
+    try:
+        sim.update_params(dt=0.005, G=1.0, epsilon=eps, theta=theta)
+        sim.sort()
+        sim.refresh()
+        sim.force()
+        acc, _ = sim.accel()
+        return acc, sim.particles()
+    finally:
+        sim.destroy()
+
+
+def bench_config(name, steps, cpu_steps, fe_sample, args):
+    """One plain-form workload: GPU steps (each step one sort + refresh + walk
+    with its own synchronisation, for p50/p99), the reference step on the host
+    cores, and the FP64 direct-sum force error."""
+    import galaxy_hip
+    from galaxy_hip import ic
+    c = ic.CONFIGS[name]
+    data = ic.make(name)
+    n = data.shape[0]
+    extra, halo = _halo_params(c)
+    log(f"{name}: N={n}, {steps} GPU steps")
+    sim = galaxy_hip.NBodyGPU(data, bucket=c["bucket"], exact_force=args.exact)
+    try:
+        sim.update_params(dt=c["dt"], G=1.0, epsilon=c["epsilon"], theta=c["theta"],
+                          haloCenters=halo, **extra)
+        sim.step(args.warmup, sort_every=1)
+        sim.sync()
+        t0 = time.perf_counter()
+        lat = _timed_steps(sim, steps)
+        total = time.perf_counter() - t0
+        inter = sim.count_interactions()
+    finally:
+        sim.destroy()
+    lat_ms = lat * 1e3
+    gpu = {"value": n * steps / total, "unit": "particle-steps/s", "steps": steps,
+           "warmup": args.warmup, "ms_per_step": total / steps * 1e3,
+           "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+           "interactions_per_step": inter}
+    log(f"{name}: GPU {gpu['value'] / 1e6:.1f} M particle-steps/s, p50 {gpu['p50_ms']:.3f} ms "
+        f"p99 {gpu['p99_ms']:.3f} ms")
+    out = {"n": n, "theta": c["theta"], "epsilon": c["epsilon"], "bucket": c["bucket"],
+           "dt": c["dt"], **({"halo": extra} if extra else {}), "gpu": gpu}
+    acc, parts = gpu_forces(data, c["bucket"], c["theta"], c["epsilon"], args.exact)
+    out["force_error_fp64"] = force_error(acc, parts, c["epsilon"], fe_sample,
+                                          0x6A1A5EED00000100 + n)
+    log(f"{name}: FP64 direct-sum error median {out['force_error_fp64']['median']:.3e} "
+        f"p99 {out['force_error_fp64']['p99']:.3e}")
+    if cpu_steps and not args.no_cpu_baseline:
+        cpu = cpu_reference_steps(data, c, cpu_steps)
+        out["cpu"] = cpu
+        out["speedup_vs_cpu"] = gpu["value"] / cpu["value"]
+        log(f"{name}: CPU reference {cpu['value'] / 1e6:.3f} M particle-steps/s on "
+            f"{cpu['cores']} threads ({cpu['step_s'] * 1e3:.1f} ms/step)")
+    return out
 
 
 def log(msg):
@@ -240,6 +352,16 @@ def bench_serve(args, tuned):
             sim.destroy()
     plain = work / t_plain
     log(f"served plain (one job at a time): {plain / 1e6:.1f} M particle-steps/s")
+    # FP64 direct-sum force error of the served force path on three jobs of the
+    # stream (the smallest, the median and the largest N) at their initial state
+    by_n = sorted(range(len(jobs)), key=lambda k: jobs[k]["n"])
+    fe = []
+    for k in (by_n[0], by_n[len(by_n) // 2], by_n[-1]):
+        acc, parts = gpu_forces(records[k], bucket, params["theta"], params["epsilon"])
+        e = force_error(acc, parts, params["epsilon"], 4096, 0x6A1A5EED00000200 + k)
+        fe.append(dict(job=jobs[k]["id"], n=jobs[k]["n"], preset=jobs[k]["preset"], **e))
+        log(f"served job {jobs[k]['id']} ({jobs[k]['preset']}, N={jobs[k]['n']}): FP64 "
+            f"direct-sum error median {e['median']:.3e} p99 {e['p99']:.3e}")
     summ.update({
         "workload": f"serve: {len(jobs)} jobs, Poisson arrivals 20 jobs/s, N log-uniform in "
                     f"[8192, 1048576], {args.serve_steps} steps per job, disk / Plummer / "
@@ -252,6 +374,7 @@ def bench_serve(args, tuned):
         "plain_one_at_a_time": plain, "speedup_vs_plain": summ["throughput"] / plain,
         "saturated_speedup_vs_plain": summ["saturated_throughput"] / plain,
         "p99_over_p50": summ["step_p99_ms"] / max(summ["step_p50_ms"], 1e-9),
+        "force_error_fp64": fe,
     })
     return summ
 
@@ -363,14 +486,31 @@ def copy_probe(mib, quiet=False):
         print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
 
 
+def bench_configs(args):
+    """The two other plain-form workloads: the 16,384-particle disk (theta 0.7,
+    eps 0.02, bucket 16) and the 500,000-particle disk with gas, halo and drag
+    (theta 0.7, eps 0.01, bucket 16, 100 steps)."""
+    return {
+        "cpu_disk_16k": bench_config("cpu_disk_16k", args.disk16k_steps, args.disk16k_cpu_steps,
+                                     None, args),
+        "disk_500k": bench_config("disk_500k", args.disk500k_steps, args.cpu_steps, 4096, args),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=4194304)
-    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="timed reference steps on the host cores (after one warm-up)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--disk16k-steps", type=int, default=200)
+    ap.add_argument("--disk16k-cpu-steps", type=int, default=20)
+    ap.add_argument("--disk500k-steps", type=int, default=100)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 16,384- and 500,000-particle plain workloads")
     ap.add_argument("--exact", action="store_true", help="IEEE sqrt/div force path")
     ap.add_argument("--lib", default=None, help="path of the libgalaxy_hip.so to load")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -381,7 +521,7 @@ def main():
     ap.add_argument("--probe-walk", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-galaxy batched run")
     ap.add_argument("--no-serve", action="store_true", help="skip the served job stream")
-    ap.add_argument("--only", choices=("headline", "batch", "serve"), default=None)
+    ap.add_argument("--only", choices=("headline", "configs", "batch", "serve"), default=None)
     ap.add_argument("--tuned", default=None, metavar="WG,BUCKET,RADIX",
                     help="skip the autotuner and use these batched-run parameters")
     ap.add_argument("--batch-steps", type=int, default=20)
@@ -410,6 +550,9 @@ def main():
 
     if args.only == "batch":
         print(json.dumps(bench_batch(args)))
+        return
+    if args.only == "configs":
+        print(json.dumps(bench_configs(args)))
         return
     if args.only == "serve":
         tuned = {}
@@ -446,6 +589,16 @@ def main():
             out["roofline"]["l2_detail"] = {
                 "hits": traffic["L2"]["hits"], "misses": traffic["L2"]["misses"],
                 "source": "rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum per walk dispatch"}
+    if not (args.no_configs or args.only == "headline"):
+        try:
+            out["configs"] = bench_configs(args)
+        except Exception as e:  # the headline line is still printed
+            out["configs"] = {"error": f"{type(e).__name__}: {e}"}
+        out["configs"]["collision_4m"] = {
+            "n": out["config"]["n"], "gpu": {k: out[k] for k in ("value", "unit", "steps", "warmup",
+                                                          "ms_per_step", "p50_ms", "p99_ms")},
+            "cpu": out["cpu_baseline"], "force_error_fp64": out["force_error_fp64"],
+            "note": "the headline above"}
     tuned = {}
     if not (args.no_batch or args.only == "headline"):
         try:
@@ -556,11 +709,20 @@ def bench_headline(args):
         "cpu_baseline": None,
     }
     log(f"headline: {out['value'] / 1e6:.1f} M particle-steps/s, walk {walk_avg_ms:.3f} ms")
+    acc, parts = gpu_forces(data, bucket, theta, eps, args.exact)
+    out["force_error_fp64"] = force_error(acc, parts, eps, 4096, 0x6A1A5EED00000100 + n)
+    del acc, parts
+    log(f"headline: FP64 direct-sum error on 4096 particles median "
+        f"{out['force_error_fp64']['median']:.3e} p99 {out['force_error_fp64']['p99']:.3e}")
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle on the host cores)")
-        cb = cpu_baseline(data, n, theta, eps, bucket, min(args.cpu_sample, n))
+        log(f"cpu baseline: {args.cpu_steps} reference steps of all {n} particles on the host")
+        c = dict(ic.CONFIGS["collision_4m"], n=n)
+        cb = cpu_reference_steps(data, c, args.cpu_steps)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"].update({k: cb[k] for k in ("step_s", "p50_s", "p99_s", "stages_s")})
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
+        log(f"cpu baseline: {cb['value'] / 1e6:.3f} M particle-steps/s ({cb['step_s']:.2f} s/step "
+            f"on {cb['cores']} threads)")
     return out
 
 

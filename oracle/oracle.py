@@ -46,6 +46,16 @@ class Params(ctypes.Structure):
     ]
 
 
+def default_threads():
+    """Worker threads for the multi-threaded walks: every core this process may
+    run on, capped by OMP_NUM_THREADS when that is set (a shared host's share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(n, 1)
+
+
 def build():
     """Compile liboracle.so with its Makefile (gcc, -ffp-contract=off)."""
     subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
@@ -239,7 +249,7 @@ def walk(particles, nodes, params, nthreads=None):
     inter = np.zeros(n, np.uint32)
     lib().or_walk(_p(particles, f32p), n, _p(nodes, u32p), ctypes.byref(params),
                   _p(pout, f32p), _p(energy, f32p), _p(acc4, f32p), _p(dens, f32p),
-                  _p(inter, u32p), nthreads or os.cpu_count() or 1)
+                  _p(inter, u32p), nthreads or default_threads())
     return pout, energy, acc4.reshape(n, 4), dens, inter
 
 
@@ -251,7 +261,7 @@ def walk_subset(particles, nodes, params, idx, nthreads=None):
     inter = np.zeros(idx.size, np.uint32)
     lib().or_walk_subset(_p(particles, f32p), n, _p(np.ascontiguousarray(nodes, np.uint32), u32p),
                          ctypes.byref(params), _p(idx, u32p), idx.size, _p(acc4, f32p),
-                         _p(inter, u32p), nthreads or os.cpu_count() or 1)
+                         _p(inter, u32p), nthreads or default_threads())
     return acc4.reshape(-1, 4), inter
 
 
@@ -286,5 +296,5 @@ def direct_sum_f64(particles, G, eps2, idx, nthreads=None):
     idx = np.ascontiguousarray(idx, dtype=np.uint32)
     acc = np.zeros(idx.size * 3, np.float64)
     lib().or_direct_sum_f64(_p(particles, f32p), n, G, eps2, _p(idx, u32p), idx.size,
-                            _p(acc, f64p), nthreads or os.cpu_count() or 1)
+                            _p(acc, f64p), nthreads or default_threads())
     return acc.reshape(-1, 3)

@@ -340,26 +340,44 @@ typedef struct {
   uint32_t begin, end;
 } ds_job;
 
+/* Samples go in tiles of DS_TILE per pass over the sources, so each source
+ * record is read once per tile; every sample's sum still runs over q in
+ * ascending order, so the results do not depend on the tiling. */
+enum { DS_TILE = 8 };
+
 static void* ds_worker(void* arg) {
   ds_job* j = (ds_job*)arg;
-  for (uint32_t k = j->begin; k < j->end; k++) {
-    uint32_t i = j->idx[k];
-    const float* pi = j->pin + (size_t)i * 8;
-    double ax = 0, ay = 0, az = 0;
-    for (uint32_t q = 0; q < j->n; q++) {
-      if (q == i) continue;
-      const float* pq = j->pin + (size_t)q * 8;
-      double dx = (double)pq[0] - pi[0], dy = (double)pq[1] - pi[1],
-             dz = (double)pq[2] - pi[2];
-      double d2 = dx * dx + dy * dy + dz * dz + j->eps2;
-      double f = j->G * pq[3] / (d2 * sqrt(d2));
-      ax += dx * f;
-      ay += dy * f;
-      az += dz * f;
+  for (uint32_t k0 = j->begin; k0 < j->end; k0 += DS_TILE) {
+    uint32_t t = j->end - k0 < DS_TILE ? j->end - k0 : DS_TILE;
+    uint32_t ii[DS_TILE];
+    double px[DS_TILE], py[DS_TILE], pz[DS_TILE];
+    double ax[DS_TILE], ay[DS_TILE], az[DS_TILE];
+    for (uint32_t u = 0; u < t; u++) {
+      ii[u] = j->idx[k0 + u];
+      const float* pi = j->pin + (size_t)ii[u] * 8;
+      px[u] = pi[0];
+      py[u] = pi[1];
+      pz[u] = pi[2];
+      ax[u] = ay[u] = az[u] = 0;
     }
-    j->acc3[(size_t)k * 3] = ax;
-    j->acc3[(size_t)k * 3 + 1] = ay;
-    j->acc3[(size_t)k * 3 + 2] = az;
+    for (uint32_t q = 0; q < j->n; q++) {
+      const float* pq = j->pin + (size_t)q * 8;
+      double qx = pq[0], qy = pq[1], qz = pq[2], mq = j->G * pq[3];
+      for (uint32_t u = 0; u < t; u++) {
+        if (q == ii[u]) continue;
+        double dx = qx - px[u], dy = qy - py[u], dz = qz - pz[u];
+        double d2 = dx * dx + dy * dy + dz * dz + j->eps2;
+        double f = mq / (d2 * sqrt(d2));
+        ax[u] += dx * f;
+        ay[u] += dy * f;
+        az[u] += dz * f;
+      }
+    }
+    for (uint32_t u = 0; u < t; u++) {
+      j->acc3[(size_t)(k0 + u) * 3] = ax[u];
+      j->acc3[(size_t)(k0 + u) * 3 + 1] = ay[u];
+      j->acc3[(size_t)(k0 + u) * 3 + 2] = az[u];
+    }
   }
   return NULL;
 }
