@@ -312,7 +312,9 @@ def bench_serve(args, tuned):
 
     import galaxy_hip
     from galaxy_hip import ic, serve
-    c = ic.CONFIGS["serve"]
+    c = dict(ic.CONFIGS["serve"])
+    if args.serve_budget:
+        c["hbm_budget"] = args.serve_budget
     jobs = ic.serve_jobs(args.serve_jobs, rate=c["rate"], lo=c["lo"], hi=c["hi"],
                          steps=args.serve_steps)
     t0 = time.perf_counter()
@@ -321,19 +323,25 @@ def bench_serve(args, tuned):
     log(f"serve: {len(jobs)} jobs generated in {time.perf_counter() - t0:.1f} s, "
         f"N total {sum(j['n'] for j in jobs)}, last arrival {jobs[-1]['arrival']:.2f} s")
     params = dict(dt=c["dt"], G=1.0, epsilon=c["epsilon"], theta=c["theta"])
-    caps = serve.slot_classes(c["lo"], c["hi"], args.serve_per_class)
     wg = tuned.get("walk_wg", 0)
     radix = tuned.get("radix_bits", 0)
     bucket = tuned.get("bucket", 16)
-    with serve.GalaxyServer(caps, hbm_budget=c["hbm_budget"],
+    t0 = time.perf_counter()
+    with serve.GalaxyServer(hbm_budget=c["hbm_budget"], max_jobs=args.serve_slots,
                             resident_limit=args.serve_resident, bucket=bucket, walk_wg=wg,
                             radix_bits=radix, keep_results=False) as srv:
+        log(f"serve: arena of {srv.pool.arena_info()['device_bytes'] / 1e9:.1f} GB for the "
+            f"{c['hbm_budget'] / 1e9:.0f} GB budget, {args.serve_slots} slots "
+            f"({time.perf_counter() - t0:.2f} s to allocate)")
         js = [serve.Job(j["id"], r, params, j["steps"], j["arrival"], bucket=bucket)
               for j, r in zip(jobs, records)]
         summ = srv.run(js, realtime=True, log=log)
     log(f"served: {summ['throughput'] / 1e6:.1f} M particle-steps/s, job-step p50 "
         f"{summ['step_p50_ms']:.2f} ms p99 {summ['step_p99_ms']:.2f} ms (server step p50 "
-        f"{summ['server_step_p50_ms']:.2f} p99 {summ['server_step_p99_ms']:.2f})")
+        f"{summ['server_step_p50_ms']:.2f} p99 {summ['server_step_p99_ms']:.2f}); queue wait "
+        f"p50 {summ['queue_wait_p50_s']:.2f} s p99 {summ['queue_wait_p99_s']:.2f} s; budget "
+        f"use peak {summ['budget_used_peak_bytes'] / 1e9:.2f} GB; held by "
+        f"{summ['jobs_held_by']}")
     # plain: every job alone on its own handle, timed over a few steps
     t_plain = 0.0
     work = 0
@@ -365,8 +373,12 @@ def bench_serve(args, tuned):
     summ.update({
         "workload": f"serve: {len(jobs)} jobs, Poisson arrivals 20 jobs/s, N log-uniform in "
                     f"[8192, 1048576], {args.serve_steps} steps per job, disk / Plummer / "
-                    "collision uniform, theta=0.5, eps=0.01, dt=0.005, 200 GB HBM budget, "
-                    f"bucket {bucket}",
+                    f"collision uniform, theta=0.5, eps=0.01, dt=0.005, "
+                    f"{c['hbm_budget'] / 1e9:.0f} GB HBM budget (arena), {args.serve_slots} "
+                    f"slots, bucket {bucket}",
+        "admission_note": "a job waits for a free slot, for the resident-particle cap "
+                          "(resident_limit, the latency bound) and for a free arena range "
+                          "(the HBM budget); jobs_held_by counts the jobs each held at least once",
         "latency_note": "step_p50/p99 count each server step once per job it advanced; "
                         "server_step_p50/p99 count each server step once",
         "value": summ["throughput"], "unit": "particle-steps/s",
@@ -532,8 +544,10 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--serve-resident", type=int, default=8 << 20,
                     help="resident particle limit of the server")
-    ap.add_argument("--serve-per-class", type=int, default=8,
-                    help="slots per power-of-two capacity class")
+    ap.add_argument("--serve-slots", type=int, default=64,
+                    help="arena slots of the server (jobs resident at once)")
+    ap.add_argument("--serve-budget", type=float, default=None,
+                    help="HBM budget of the server in bytes (default: the workload's 200 GB)")
     args = ap.parse_args()
     if args.lib:
         os.environ["GALAXY_HIP_LIB"] = os.path.abspath(args.lib)

@@ -274,7 +274,7 @@ void mark_refreshed(gs_sim* s) {
 
 void mark_stepped(gs_sim* s) {
   for (auto& h : s->hs) {
-    h.have_step = h.d.n > 0;
+    h.have_step = h.have_red = h.d.n > 0;
     h.have_force = false;
   }
 }
@@ -358,10 +358,161 @@ void run_step(gs_sim* s, bool sort) {
 
 uint32_t align_up(uint64_t v, uint32_t a) { return (uint32_t)((v + a - 1) / a * a); }
 
+void push_tables(gs_sim* s, const std::vector<uint2>& chunk_tab,
+                 const std::vector<uint32_t>& tile_tab, const std::vector<uint32_t>& wblock_slot) {
+  if (!chunk_tab.empty())
+    check(hipMemcpyAsync(s->chunk_tab, chunk_tab.data(), sizeof(uint2) * chunk_tab.size(),
+                         hipMemcpyHostToDevice, s->stream),
+          "tables");
+  if (!tile_tab.empty())
+    check(hipMemcpyAsync(s->tile_tab, tile_tab.data(), sizeof(uint32_t) * tile_tab.size(),
+                         hipMemcpyHostToDevice, s->stream),
+          "tables");
+  if (!wblock_slot.empty())
+    check(hipMemcpyAsync(s->wblock_slot, wblock_slot.data(), sizeof(uint32_t) * wblock_slot.size(),
+                         hipMemcpyHostToDevice, s->stream),
+          "tables");
+}
+
+// Device bytes of a pool: P particle records, C node records, the per-block
+// arrays and tables at their allocated sizes, the per-slot records, stellar
+// records when present and the node staging.
+uint64_t pool_bytes(uint64_t P, uint64_t C, uint64_t chunks, uint64_t build_words,
+                    uint64_t walk_blocks, uint32_t nslots, bool stellar, uint64_t node64) {
+  return P * GS_ARENA_PARTICLE_BYTES + C * GS_ARENA_NODE_BYTES +
+         2 * build_words * 8 +                   // build look-back words (ping-pong)
+         chunks * (GS_HIST_WORDS * 4 + 8 * 4 +   // sort look-back words, bbox partials
+                   sizeof(uint2) + 4) +          // chunk and tile tables
+         walk_blocks * (4 + GS_NRED * 8) +       // walk block table, reduction partials
+         (P / GS_BIG_LEAF + 64) * sizeof(uint2) +  // large-leaf list
+         (uint64_t)nslots * (sizeof(DevSlot) + sizeof(DevScalars) + sizeof(DevParams) + 4) +
+         sizeof(DevBatch) + GS_NRED * 8 + (stellar ? P * 32 : 0) + node64 * 64;
+}
+
+// node download / upload staging for pools of up to ncap nodes
+void ensure_node64(gs_sim* s, uint64_t ncap) {
+  if (s->node64 && s->node64_cap >= ncap) return;
+  if (s->node64) {
+    sync(s);
+    (void)hipFree(s->node64);
+    s->node64 = nullptr;
+    s->node64_cap = 0;
+  }
+  dalloc(&s->node64, (size_t)ncap * 16, "node staging");
+  s->node64_cap = ncap;
+}
+
+// Rebuilds an arena's launch tables over its occupied slots (slot order):
+// bbox/encode chunks = sort tiles, build look-back words and walk blocks of
+// each slot follow those of the slots before it.  The build's look-back words
+// start cleared, the step graphs (captured with the old launch sizes) are
+// dropped, and the reduction partials of the last step no longer sit where
+// the slots' walk blocks now point.
+void apply_layout(gs_sim* s) {
+  std::vector<uint2> chunk_tab;
+  std::vector<uint32_t> tile_tab, wblock_slot;
+  uint32_t chunk = 0, bt = 0, wb = 0;
+  const uint32_t wg = s->walk_wg;
+  for (uint32_t k = 0; k < s->nslots; k++) {
+    DevSlot& d = s->hs[k].d;
+    d.nchunks = (d.cap + GS_CHUNK - 1) / GS_CHUNK;
+    d.chunk0 = chunk;
+    d.tile0 = chunk;
+    for (uint32_t c = 0; c < d.nchunks; c++) {
+      chunk_tab.push_back(make_uint2(k, c * GS_CHUNK));
+      tile_tab.push_back((k << 16) | c);
+    }
+    chunk += d.nchunks;
+    d.btoff = bt;
+    bt += (d.ncap + GS_BUILD_TILE - 1) / GS_BUILD_TILE;
+    d.wblock0 = wb;
+    d.nwblocks = align_up(d.cap, GS_SLOT_ALIGN) / wg;
+    for (uint32_t b = 0; b < d.nwblocks; b++) wblock_slot.push_back(k);
+    wb += d.nwblocks;
+    s->hs[k].have_red = false;
+  }
+  need(chunk <= s->cap_chunks && bt <= s->cap_build_words && wb <= s->cap_walk_blocks,
+       GS_E_INVALID, "arena launch tables overflow");
+  s->chunks = chunk;
+  s->sort_tiles = chunk;
+  s->build_words = bt;
+  s->walk_blocks = wb;
+  push_tables(s, chunk_tab, tile_tab, wblock_slot);
+  std::vector<DevSlot> ds(s->nslots);
+  for (uint32_t k = 0; k < s->nslots; k++) ds[k] = s->hs[k].d;
+  check(hipMemcpyAsync(s->slots, ds.data(), sizeof(DevSlot) * s->nslots, hipMemcpyHostToDevice,
+                       s->stream),
+        "slot descriptors");
+  // the build's look-back words of the new layout start cleared (each level
+  // clears the other buffer for the next); the sort's are cleared by every
+  // step's first kernel
+  for (int k = 0; k < 2; k++)
+    if (bt)
+      check(gs_launch_fill_u32((uint32_t*)s->build_status[k], 0, 2ull * bt, s->stream), "fill");
+  sync(s);
+  destroy_graphs(s);
+  s->layout_dirty = false;
+}
+
+// Before every launch over all slots: an arena's tables follow its slots.
+// False when no slot holds a galaxy (an empty arena has nothing to launch).
+bool ready(gs_sim* s) {
+  if (s->arena && s->layout_dirty) apply_layout(s);
+  return s->walk_blocks > 0;
+}
+
+// An arena slot's ranges back to the free lists; the slot holds nothing.
+void arena_release(gs_sim* s, HostSlot& h) {
+  if (!h.d.cap) return;
+  const bool ok = s->pfree.give(h.d.poff / GS_SLOT_ALIGN,
+                                align_up(h.d.cap, GS_SLOT_ALIGN) / GS_SLOT_ALIGN) &&
+                  s->nfree.give(h.d.noff, h.d.ncap);
+  need(ok, GS_E_INVALID, "arena free lists corrupt");
+  // the slot's tree scalars go to zero (no nodes, no levels): a refresh or
+  // walk before its next rebuild finds nothing to do in the ranges given away
+  const size_t k = (size_t)(&h - s->hs.data());
+  check(gs_launch_fill_u32((uint32_t*)(s->sc + k), 0, sizeof(DevScalars) / 4, s->stream),
+        "clear slot scalars");
+  h.d.cap = h.d.ncap = h.d.poff = h.d.noff = 0;
+  h.d.max_nodes = 0;
+  h.d.nchunks = h.d.nwblocks = 0;  // no blocks until the tables are rebuilt
+  s->layout_dirty = true;
+}
+
+// Ranges for n particles in an arena slot: align_up(n, GS_SLOT_ALIGN)
+// particles and 2n + 4096 nodes (mortonTree.js:88), best fit; a slot that
+// already holds n keeps its ranges.  GS_E_FULL (the slot left empty) when
+// no free range holds them.
+void arena_place(gs_sim* s, HostSlot& h, uint64_t n) {
+  if (n && n == h.d.cap) return;
+  arena_release(s, h);
+  if (!n) return;
+  need(n <= GS_MAX_N, GS_E_INVALID, "more particles than a slot may hold");
+  const uint64_t units = align_up(n, GS_SLOT_ALIGN) / GS_SLOT_ALIGN;
+  const uint64_t nodes = std::min<uint64_t>(2 * n + 4096, GS_NODE32_MAX);
+  uint64_t pu = 0, nu = 0;
+  need(s->pfree.take(units, &pu), GS_E_FULL, "arena full: no free particle range holds the galaxy");
+  if (!s->nfree.take(nodes, &nu)) {
+    s->pfree.give(pu, units);
+    throw Fail{GS_E_FULL, "arena full: no free node range holds the galaxy's tree"};
+  }
+  h.d.cap = (uint32_t)n;
+  h.d.poff = (uint32_t)(pu * GS_SLOT_ALIGN);
+  h.d.noff = (uint32_t)nu;
+  h.d.ncap = (uint32_t)nodes;
+  h.d.max_nodes = nodes;
+  h.d.nchunks = h.d.nwblocks = 0;
+  s->max_ncap = std::max(s->max_ncap, h.d.ncap);
+  s->layout_dirty = true;
+}
+
 // Allocates a pool of nslots slots (caps[k] particles, buckets[k] leaf size,
-// a node pool of max(2 caps[k] + 4096, max_nodes) nodes).
+// a node pool of max(2 caps[k] + 4096, max_nodes) nodes), or, with
+// arena_p > 0, an arena of arena_p particles (a multiple of GS_SLOT_ALIGN)
+// and arena_c nodes whose nslots slots start empty.
 gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots, uint32_t flags,
-                  int device, uint32_t wg, uint32_t radix_bits, uint64_t max_nodes) {
+                  int device, uint32_t wg, uint32_t radix_bits, uint64_t max_nodes,
+                  uint64_t arena_p = 0, uint64_t arena_c = 0) {
   gs_sim* s = new gs_sim();
   s->flags = flags;
   s->walk_wg = wg;
@@ -376,6 +527,8 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
   for (uint32_t k = 0; k < nslots; k++) {
     DevSlot& d = s->hs[k].d;
     memset(&d, 0, sizeof d);
+    d.bucket = buckets && buckets[k] ? buckets[k] : 16;
+    if (arena_p) continue;  // arena slots get their ranges when filled
     const uint32_t cap = caps[k];
     d.n = 0;
     d.cap = cap;
@@ -384,7 +537,6 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     d.ncap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(2ull * cap + 4096, max_nodes),
                                           GS_NODE32_MAX);
     d.noff = (uint32_t)noff;
-    d.bucket = buckets && buckets[k] ? buckets[k] : 16;
     d.max_nodes = d.ncap;
     d.nchunks = (cap + GS_CHUNK - 1) / GS_CHUNK;
     d.chunk0 = chunk;
@@ -412,6 +564,22 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
   s->sort_tiles = tile;
   s->build_words = bt;
   s->walk_blocks = wb;
+  s->cap_chunks = chunk;
+  s->cap_build_words = bt;
+  s->cap_walk_blocks = wb;
+  if (arena_p) {
+    // an arena's slots take ranges of the arena: each slot's blocks are at
+    // most its aligned span / GS_CHUNK + 1 chunks, ncap / GS_BUILD_TILE + 1
+    // look-back words and span / wg walk blocks
+    s->arena = true;
+    s->ptot = arena_p;
+    s->ntot = arena_c;
+    s->cap_chunks = (uint32_t)(arena_p / GS_CHUNK + nslots);
+    s->cap_build_words = (uint32_t)(arena_c / GS_BUILD_TILE + nslots);
+    s->cap_walk_blocks = (uint32_t)(arena_p / wg);
+    s->pfree.reset(arena_p / GS_SLOT_ALIGN);
+    s->nfree.reset(arena_c);
+  }
   int rc = guarded(s, [&] {
     need(s->ptot < (1ull << 32) && s->ntot < (1ull << 32), GS_E_INVALID,
          "pool larger than 2^32 particles or nodes");
@@ -434,11 +602,11 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
       dalloc(&s->velp[k], P, "velp");
       dalloc(&s->keys[k], P, "keys");
       dalloc(&s->vals[k], P, "vals");
-      dalloc(&s->build_status[k], s->build_words, "build status");
+      dalloc(&s->build_status[k], s->cap_build_words, "build status");
     }
     dalloc(&s->codes_raw, P, "codes");
-    dalloc(&s->sort_status, (size_t)GS_HIST_WORDS * s->sort_tiles, "sort status");
-    dalloc(&s->bbox_part, (size_t)s->chunks * 8, "bbox");
+    dalloc(&s->sort_status, (size_t)GS_HIST_WORDS * s->cap_chunks, "sort status");
+    dalloc(&s->bbox_part, (size_t)s->cap_chunks * 8, "bbox");
     dalloc(&s->nodes, s->ntot, "nodes");
     dalloc(&s->nsize, s->ntot, "node sizes");
     dalloc(&s->massless, nslots, "massless flags");
@@ -446,15 +614,15 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     dalloc(&s->acc, P, "acc");
     dalloc(&s->dens, P, "density");
     dalloc(&s->inter, P, "interactions");
-    dalloc(&s->red_part, (size_t)s->walk_blocks * GS_NRED, "reductions");
+    dalloc(&s->red_part, (size_t)s->cap_walk_blocks * GS_NRED, "reductions");
     dalloc(&s->red_out, GS_NRED, "reduction");
     dalloc(&s->slots, nslots, "slots");
     dalloc(&s->sc, nslots, "scalars");
     dalloc(&s->prm, nslots, "params");
     dalloc(&s->batch, 1, "batch");
-    dalloc(&s->chunk_tab, s->chunks, "chunk table");
-    dalloc(&s->tile_tab, s->sort_tiles, "tile table");
-    dalloc(&s->wblock_slot, s->walk_blocks, "walk block table");
+    dalloc(&s->chunk_tab, s->cap_chunks, "chunk table");
+    dalloc(&s->tile_tab, s->cap_chunks, "tile table");
+    dalloc(&s->wblock_slot, s->cap_walk_blocks, "walk block table");
     // a leaf of more than GS_BIG_LEAF members holds that many particles
     s->big_cap = (uint32_t)(P / GS_BIG_LEAF + 64);
     dalloc(&s->big_leaf, s->big_cap, "large-leaf list");
@@ -465,21 +633,13 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     check(gs_launch_fill_u32(s->massless, 1u, nslots, s->stream), "fill");
     check(gs_launch_fill_u32((uint32_t*)s->energy, 0, P, s->stream), "fill");
     for (int k = 0; k < 2; k++) {
-      check(gs_launch_fill_u32((uint32_t*)s->build_status[k], 0, 2ull * s->build_words,
+      check(gs_launch_fill_u32((uint32_t*)s->build_status[k], 0, 2ull * s->cap_build_words,
                                s->stream),
             "fill");
       check(gs_launch_fill_u32((uint32_t*)s->posm[k], 0, 4 * P, s->stream), "fill");
       check(gs_launch_fill_u32((uint32_t*)s->velp[k], 0, 4 * P, s->stream), "fill");
     }
-    check(hipMemcpyAsync(s->chunk_tab, chunk_tab.data(), sizeof(uint2) * s->chunks,
-                         hipMemcpyHostToDevice, s->stream),
-          "tables");
-    check(hipMemcpyAsync(s->tile_tab, tile_tab.data(), sizeof(uint32_t) * s->sort_tiles,
-                         hipMemcpyHostToDevice, s->stream),
-          "tables");
-    check(hipMemcpyAsync(s->wblock_slot, wblock_slot.data(), sizeof(uint32_t) * s->walk_blocks,
-                         hipMemcpyHostToDevice, s->stream),
-          "tables");
+    push_tables(s, chunk_tab, tile_tab, wblock_slot);
     gs_params p;
     gs_params_default(&p);
     for (uint32_t k = 0; k < nslots; k++) {
@@ -524,6 +684,7 @@ DevScalars read_scalars(gs_sim* s, uint32_t slot) {
 void upload_particles(gs_sim* s, uint32_t slot, const float* particles, uint64_t n) {
   HostSlot& h = s->hs[slot];
   need(particles != nullptr || n == 0, GS_E_INVALID, "null particles");
+  if (s->arena) arena_place(s, h, n);
   need(n <= h.d.cap, GS_E_INVALID, "more particles than the slot holds");
   check(gs_upload_records(s, slot, particles, (uint32_t)n), "upload particles");
   if (s->stel[0])  // a new galaxy starts without stellar records until it uploads its own
@@ -536,7 +697,7 @@ void upload_particles(gs_sim* s, uint32_t slot, const float* particles, uint64_t
   push_slot(s, slot);
   push_params(s, slot);
   sync(s);
-  h.have_tree = h.have_force = h.have_step = h.have_codes = h.have_com = false;
+  h.have_tree = h.have_force = h.have_step = h.have_codes = h.have_com = h.have_red = false;
 }
 
 // Stellar records are allocated on first upload; step graphs captured
@@ -657,13 +818,69 @@ const char* gs_last_error(const gs_sim* s) { return s ? s->err.c_str() : "null h
 int gs_pool_info(gs_sim* s, uint32_t* nslots, uint64_t* device_bytes) {
   return guarded(s, [&] {
     if (nslots) *nslots = s->nslots;
-    if (device_bytes) {
-      const uint64_t P = s->ptot;
-      *device_bytes = 2 * P * 32 + 4 * P * 4 + P * 4 + P * (4 + 16 + 4 + 4) +
-                      (uint64_t)s->ntot * (sizeof(NodeRec) + sizeof(float)) +
-                      2ull * s->build_words * 8 +
-                      (uint64_t)GS_HIST_WORDS * s->sort_tiles * 4 +
-                      (uint64_t)s->walk_blocks * GS_NRED * 8;
+    if (device_bytes)
+      *device_bytes = pool_bytes(s->ptot, s->ntot, s->cap_chunks, s->cap_build_words,
+                                 s->cap_walk_blocks, s->nslots, s->stel[0] != nullptr,
+                                 s->node64 ? s->node64_cap : 0);
+  });
+}
+
+int gs_create_arena(const gs_arena_config* cfg, gs_sim** out) {
+  if (!cfg || !out) return GS_E_INVALID;
+  *out = nullptr;
+  if (cfg->nslots == 0 || cfg->nslots > GS_MAX_SLOTS) return GS_E_INVALID;
+  const uint32_t wg = cfg->walk_wg ? cfg->walk_wg : 64;
+  if (!valid_wg(wg)) return GS_E_INVALID;
+  if (cfg->radix_bits != 0 && cfg->radix_bits != 8 && cfg->radix_bits != 10) return GS_E_INVALID;
+  // the most particles P (a multiple of GS_SLOT_ALIGN) whose arena, with
+  // C = 2P + 4096 nslots nodes (every slot's 2n + 4096 fits), stays within
+  // the budget; P and C stay below 2^32
+  const uint64_t ns = cfg->nslots;
+  auto bytes = [&](uint64_t P) {
+    const uint64_t C = 2 * P + 4096 * ns;
+    return pool_bytes(P, C, P / GS_CHUNK + ns, C / GS_BUILD_TILE + ns, P / wg, cfg->nslots, false,
+                      0);
+  };
+  const uint64_t per = GS_ARENA_PARTICLE_BYTES + 2 * GS_ARENA_NODE_BYTES;
+  const uint64_t pmax = ((0xffffffffull - 4096 * ns) / 2) / GS_SLOT_ALIGN * GS_SLOT_ALIGN;
+  uint64_t P = std::min<uint64_t>(cfg->budget_bytes / per / GS_SLOT_ALIGN * GS_SLOT_ALIGN, pmax);
+  while (P && bytes(P) > cfg->budget_bytes) {
+    const uint64_t step = ((bytes(P) - cfg->budget_bytes) / per / GS_SLOT_ALIGN + 1) * GS_SLOT_ALIGN;
+    P = P > step ? P - step : 0;
+  }
+  if (!P) return GS_E_INVALID;  // the budget holds no aligned range
+  gs_sim* s = nullptr;
+  try {
+    s = make_pool(nullptr, nullptr, cfg->nslots, cfg->flags, cfg->device, wg,
+                  cfg->radix_bits ? cfg->radix_bits : 8, 0, P, 2 * P + 4096 * ns);
+  } catch (...) {
+    s = nullptr;
+  }
+  if (!s) return GS_E_DEVICE;
+  *out = s;
+  return GS_OK;
+}
+
+int gs_arena_info(gs_sim* s, uint64_t* device_bytes, uint64_t* used_bytes, uint64_t* largest_fit) {
+  return guarded(s, [&] {
+    need(s->arena, GS_E_INVALID, "not an arena pool");
+    if (device_bytes)
+      *device_bytes = pool_bytes(s->ptot, s->ntot, s->cap_chunks, s->cap_build_words,
+                                 s->cap_walk_blocks, s->nslots, false,
+                                 s->node64 ? s->node64_cap : 0);
+    if (used_bytes) {
+      uint64_t u = 0;
+      for (const auto& h : s->hs)
+        if (h.d.cap)
+          u += (uint64_t)align_up(h.d.cap, GS_SLOT_ALIGN) * GS_ARENA_PARTICLE_BYTES +
+               (uint64_t)h.d.ncap * GS_ARENA_NODE_BYTES;
+      *used_bytes = u;
+    }
+    if (largest_fit) {
+      const uint64_t by_p = s->pfree.largest() * GS_SLOT_ALIGN;
+      const uint64_t nl = s->nfree.largest();
+      const uint64_t by_n = nl >= 4096 ? (nl - 4096) / 2 : 0;
+      *largest_fit = std::min<uint64_t>(std::min(by_p, by_n), GS_MAX_N);
     }
   });
 }
@@ -741,6 +958,7 @@ int gs_update_central_mass(gs_sim* s, float mass, uint32_t idx) {
 
 int gs_apply_permutation(gs_sim* s, const uint32_t* perm, uint64_t n) {
   return guarded(s, [&] {
+    need(!s->arena, GS_E_INVALID, "applyPermutation needs a plain handle, not an arena");
     const HostSlot& h = s->hs[0];
     need(perm != nullptr && n == h.d.n, GS_E_INVALID, "permutation needs N entries");
     std::vector<uint8_t> seen(n, 0);
@@ -764,6 +982,7 @@ int gs_apply_permutation(gs_sim* s, const uint32_t* perm, uint64_t n) {
 int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
                    const uint32_t* level_ranges, uint32_t num_levels) {
   return guarded(s, [&] {
+    need(!s->arena, GS_E_INVALID, "tree uploads need a plain handle, not an arena");
     HostSlot& h = s->hs[0];
     need(nodes != nullptr && num_nodes >= 1 && num_nodes <= h.d.ncap, GS_E_INVALID,
          "node count outside [1, pool]");
@@ -814,7 +1033,7 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
       }
       need(next == num_nodes, GS_E_INVALID, "level ranges must cover every node");
     }
-    if (!s->node64) dalloc(&s->node64, (size_t)s->max_ncap * 16, "node staging");
+    ensure_node64(s, s->max_ncap);
     check(hipMemcpyAsync(s->node64, nodes, num_nodes * 64, hipMemcpyHostToDevice, s->stream),
           "nodes");
     DevScalars sc = read_scalars(s, 0);
@@ -844,6 +1063,7 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
 
 int gs_encode(gs_sim* s) {
   return guarded(s, [&] {
+    if (!ready(s)) return;
     Stage t(s, GS_T_ENCODE);
     check(gs_launch_bbox(s, s->posm[s->cur]), "bbox");
     check(gs_launch_root(s), "root");
@@ -857,6 +1077,7 @@ int gs_encode(gs_sim* s) {
 
 int gs_sort(gs_sim* s) {
   return guarded(s, [&] {
+    if (!ready(s)) return;
     launch_sort(s, true);
     mark_sorted(s);
     for (auto& h : s->hs) h.have_codes = h.d.n > 0;
@@ -866,6 +1087,7 @@ int gs_sort(gs_sim* s) {
 int gs_refresh(gs_sim* s) {
   return guarded(s, [&] {
     for (auto& h : s->hs) need(!h.d.n || h.have_tree, GS_E_STATE, "refresh before any sort");
+    if (!ready(s)) return;
     launch_refresh(s);
     mark_refreshed(s);
   });
@@ -874,6 +1096,7 @@ int gs_refresh(gs_sim* s) {
 int gs_force(gs_sim* s) {
   return guarded(s, [&] {
     need_com(s, "force needs a tree with COMs (sort, then refresh or upload a tree)");
+    if (!ready(s)) return;
     Stage t(s, GS_T_WALK);
     check(gs_launch_walk(s, false, false), "walk");
     for (auto& h : s->hs) h.have_force = h.d.n > 0;
@@ -882,6 +1105,7 @@ int gs_force(gs_sim* s) {
 
 int gs_step(gs_sim* s, uint32_t nsteps, uint32_t sort_every) {
   return guarded(s, [&] {
+    if (!ready(s)) return;
     for (uint32_t k = 0; k < nsteps; k++) {
       bool missing = false;  // a slot without a tree forces a rebuild
       for (auto& h : s->hs) missing |= h.d.n && !h.have_tree;
@@ -896,6 +1120,7 @@ int gs_step(gs_sim* s, uint32_t nsteps, uint32_t sort_every) {
 int gs_walk_step(gs_sim* s) {
   return guarded(s, [&] {
     need_com(s, "step needs a tree with COMs (refresh, upload a tree or gs_step first)");
+    if (!ready(s)) return;
     if (use_graph(s)) {
       Stage w(s, GS_T_WALK);
       check(hipGraphLaunch(step_graph(s, false, 2), s->stream), "graph launch");
@@ -997,7 +1222,7 @@ int gs_slot_download_nodes(gs_sim* s, uint32_t slot, uint32_t* dst, uint64_t cap
     const uint32_t nn = read_scalars(s, slot).num_nodes;
     if (num_nodes) *num_nodes = nn;
     need(dst != nullptr && cap >= nn, GS_E_INVALID, "node buffer too small");
-    if (!s->node64) dalloc(&s->node64, (size_t)s->max_ncap * 16, "node staging");
+    ensure_node64(s, hs.d.ncap);
     check(gs_launch_node64(s, slot), "node64");
     check(hipMemcpyAsync(dst, s->node64, (size_t)nn * 64, hipMemcpyDeviceToHost, s->stream),
           "nodes");
@@ -1036,6 +1261,7 @@ int gs_slot_reduce(gs_sim* s, uint32_t slot, double out[GS_NRED]) {
     const HostSlot& h = slot_of(s, slot);
     need(out != nullptr, GS_E_INVALID, "null output");
     need(h.have_step, GS_E_STATE, "no step yet");
+    need(h.have_red, GS_E_STATE, "slots changed since the last step: its reductions are gone");
     check(gs_launch_reduce_final(s, slot, s->red_out), "reduce");
     check(hipMemcpyAsync(out, s->red_out, sizeof(double) * GS_NRED, hipMemcpyDeviceToHost,
                          s->stream),
@@ -1050,6 +1276,7 @@ int gs_upload_stellar(gs_sim* s, const float* stellar4, uint64_t n) {
 
 int gs_slot_upload_stellar(gs_sim* s, uint32_t slot, const float* stellar4, uint64_t n) {
   return guarded(s, [&] {
+    need(!s->arena, GS_E_INVALID, "an arena holds no stellar records");
     const HostSlot& h = slot_of(s, slot);
     need(stellar4 != nullptr && n == h.d.n, GS_E_INVALID,
          "stellar records must match the slot's particles (n x 4 f32)");
@@ -1103,6 +1330,7 @@ int gs_slot_locate(gs_sim* s, uint32_t slot, uint32_t old_index, uint32_t* new_i
 int gs_star_formation(gs_sim* s) {
   return guarded(s, [&] {
     need(s->stel[0] != nullptr, GS_E_STATE, "star formation needs stellar records");
+    if (!ready(s)) return;
     check(gs_launch_star_formation(s), "star formation");
   });
 }
@@ -1124,6 +1352,8 @@ int gs_stage_times(gs_sim* s, double ms[GS_NT], uint64_t counts[GS_NT], int rese
 int gs_count_interactions(gs_sim* s, uint64_t* total, uint32_t* per_particle) {
   return guarded(s, [&] {
     need_com(s, "counting needs a tree with COMs (sort, then refresh or upload a tree)");
+    if (total) *total = 0;
+    if (!ready(s)) return;
     uint64_t zero = 0;
     check(hipMemcpyAsync(&s->batch->interactions, &zero, 8, hipMemcpyHostToDevice, s->stream),
           "zero");
@@ -1144,7 +1374,7 @@ int gs_compute_morton_codes(gs_sim* s, const float* pos, uint32_t n, uint32_t st
     HostSlot& h = s->hs[0];
     need(pos && codes && n >= 1 && n <= h.d.cap && stride >= 3, GS_E_INVALID,
          "bad morton arguments");
-    need(s->nslots == 1, GS_E_INVALID, "host-array functions need a one-slot handle");
+    need(s->nslots == 1 && !s->arena, GS_E_INVALID, "host-array functions need a one-slot handle");
     // strided host positions -> posm of the current buffer (mass unused)
     std::vector<float> tmp((size_t)n * 4);
     for (size_t i = 0; i < n; i++) {
@@ -1177,7 +1407,7 @@ int gs_radix_sort_indices(gs_sim* s, const uint32_t* codes, uint32_t n, uint32_t
   return guarded(s, [&] {
     HostSlot& h = s->hs[0];
     need(codes && perm && n >= 1 && n <= h.d.cap, GS_E_INVALID, "bad sort arguments");
-    need(s->nslots == 1, GS_E_INVALID, "host-array functions need a one-slot handle");
+    need(s->nslots == 1 && !s->arena, GS_E_INVALID, "host-array functions need a one-slot handle");
     h.d.n = n;
     push_slot(s, 0);
     check(hipMemcpyAsync(s->keys[0] + h.d.poff, codes, 4ull * n, hipMemcpyHostToDevice, s->stream),
@@ -1200,7 +1430,7 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
     HostSlot& h = s->hs[0];
     need(sorted_codes && pos3 && masses && n >= 1 && n <= h.d.cap, GS_E_INVALID,
          "bad build arguments");
-    need(s->nslots == 1, GS_E_INVALID, "host-array functions need a one-slot handle");
+    need(s->nslots == 1 && !s->arena, GS_E_INVALID, "host-array functions need a one-slot handle");
     std::vector<float> pm((size_t)n * 4);
     for (size_t i = 0; i < n; i++) {
       pm[4 * i] = pos3[3 * i];
@@ -1259,7 +1489,7 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
       }
     if (nodes) {
       need(cap >= r.num_nodes, GS_E_INVALID, "node buffer too small");
-      if (!s->node64) dalloc(&s->node64, (size_t)s->max_ncap * 16, "node staging");
+      ensure_node64(s, s->max_ncap);
       check(gs_launch_node64(s, 0), "node64");
       check(hipMemcpyAsync(nodes, s->node64, (size_t)r.num_nodes * 64, hipMemcpyDeviceToHost,
                            s->stream),

@@ -439,20 +439,24 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict_
                                                      uint32_t target,
                                                      const DevSlot* __restrict__ slots,
                                                      const uint32_t* __restrict__ wblock_slot,
-                                                     uint32_t wg, uint64_t ptot,
+                                                     uint32_t wg, uint64_t total,
                                                      const float4* __restrict__ pin,
                                                      const float4* __restrict__ qin,
                                                      float4* __restrict__ pout,
                                                      float4* __restrict__ qout,
                                                      const float4* __restrict__ sin,
                                                      float4* __restrict__ sout) {
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < ptot;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t slot = wblock_slot[p / wg];
+  // q runs over the walk blocks' particles (each slot's aligned span, slot
+  // after slot in table order)
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t blk = (uint32_t)(q / wg);
+    const uint32_t slot = wblock_slot[blk];
     const DevSlot sl = slots[slot];
-    const uint32_t local = (uint32_t)(p - sl.poff);
+    const uint32_t local = (blk - sl.wblock0) * wg + (uint32_t)(q % wg);
     uint32_t src = local;
     if (slot == target && local < sl.n) src = perm[local];
+    const uint64_t p = (uint64_t)sl.poff + local;
     pout[p] = pin[sl.poff + src];
     qout[p] = qin[sl.poff + src];
     if (sin) sout[p] = sin[sl.poff + src];
@@ -490,10 +494,12 @@ hipError_t gs_launch_import64(gs_sim* s, uint32_t slot, const uint32_t* words, c
 }
 
 hipError_t gs_launch_gather(gs_sim* s, uint32_t slot, const uint32_t* perm) {
-  uint64_t g = (s->ptot + 255) / 256;
+  const uint64_t total = (uint64_t)s->walk_blocks * s->walk_wg;
+  uint64_t g = (total + 255) / 256;
   if (g > 8192) g = 8192;
+  if (g < 1) return hipSuccess;
   gather_kernel<<<(uint32_t)g, 256, 0, s->stream>>>(perm, slot, s->slots, s->wblock_slot,
-                                                     s->walk_wg, s->ptot, s->posm[s->cur],
+                                                     s->walk_wg, total, s->posm[s->cur],
                                                      s->velp[s->cur], s->posm[s->cur ^ 1],
                                                      s->velp[s->cur ^ 1], s->stel[s->stel_cur],
                                                      s->stel[s->stel_cur ^ 1]);

@@ -47,6 +47,7 @@
 #include <vector>
 
 #include "../../include/galaxy_hip.h"
+#include "arena.h"
 
 #define GS_MAX_LEVEL 10   // mortonTree.js:26
 #define GS_MAX_LEVELS 16  // level slots (NBodyGPU.js:171-174)
@@ -285,6 +286,9 @@ struct HostSlot {
   // step ran since the last topology build, which writes only the info words)
   bool have_com = false;
   bool fixed_max_nodes = false;  // else the leaf rule's pool follows n: 2n + 4096
+  // the per-block reduction partials of the last step are where this slot's
+  // walk blocks put them (an arena's launch tables were not rebuilt since)
+  bool have_red = false;
 };
 
 struct gs_sim {
@@ -293,13 +297,25 @@ struct gs_sim {
   uint32_t flags = 0, walk_wg = 256;
   uint32_t radix_bits = 8;  // sort digit width: 8 or 10
   uint32_t nslots = 1;
-  uint64_t ptot = 0;         // sum of slot capacities (aligned)
-  uint64_t ntot = 0;         // sum of node pools
+  uint64_t ptot = 0;         // particle records (sum of aligned slot capacities, or the arena's)
+  uint64_t ntot = 0;         // node records (sum of node pools, or the arena's)
   uint32_t chunks = 0;       // bbox/encode blocks
   uint32_t sort_tiles = 0;   // sum over slots of capacity tiles
   uint32_t build_words = 0;  // look-back words per build buffer
   uint32_t walk_blocks = 0;
   uint32_t max_ncap = 0;
+  // allocated sizes of the per-block arrays and tables (the launch sizes above
+  // stay within them; equal to them for a pool of fixed slots)
+  uint32_t cap_chunks = 0, cap_build_words = 0, cap_walk_blocks = 0;
+  uint64_t node64_cap = 0;  // nodes the download / upload staging holds
+
+  // arena pools: slot ranges come from these free lists (particles in units
+  // of GS_SLOT_ALIGN, nodes in nodes) when a slot is filled and go back when
+  // it is emptied; the launch tables cover the occupied slots and are rebuilt
+  // before the next launch after a change (layout_dirty)
+  bool arena = false;
+  bool layout_dirty = false;
+  gs::RangeList pfree, nfree;
 
   float4* posm[2] = {nullptr, nullptr};
   float4* velp[2] = {nullptr, nullptr};

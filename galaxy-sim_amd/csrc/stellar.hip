@@ -54,10 +54,12 @@ __global__ __launch_bounds__(256) void star_formation_kernel(
     float4* __restrict__ velp_all, float4* __restrict__ stel_all,
     const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot, uint32_t wg,
     const DevParams* __restrict__ prms) {
-  const uint32_t blk = blockIdx.x * blockDim.x / wg;  // walk-block table: wg particles each
+  // the walk blocks' particles: wg per block, each slot's aligned span
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t blk = q / wg;
   const uint32_t slot = wblock_slot[blk];
   const DevSlot sl = slots[slot];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x - sl.poff;  // slot-local index
+  const uint32_t i = (blk - sl.wblock0) * wg + q % wg;  // slot-local index
   if (i >= sl.n) return;
   const DevParams& prm = prms[slot];
   float4* velp = velp_all + sl.poff;
@@ -95,7 +97,9 @@ __global__ __launch_bounds__(256) void star_formation_kernel(
 
 hipError_t gs_launch_star_formation(gs_sim* s) {
   if (!s->stel[0]) return hipSuccess;
-  const uint32_t blocks = (uint32_t)(s->ptot / 256);  // ptot is a multiple of 1024
+  // the walk blocks cover whole aligned spans (multiples of 1024 particles)
+  const uint32_t blocks = (uint32_t)((uint64_t)s->walk_blocks * s->walk_wg / 256);
+  if (!blocks) return hipSuccess;
   star_formation_kernel<<<blocks, 256, 0, s->stream>>>(
       s->velp[s->cur], s->stel[s->stel_cur], s->slots, s->wblock_slot,
       s->walk_wg, s->prm);

@@ -4,6 +4,8 @@ C ABI of libgalaxy_hip.so (include/galaxy_hip.h).
   NBodyGPU            src/physics/NBodyGPU.js:14-369 (one galaxy on one GPU)
   GalaxyPool          many galaxies in fixed-capacity slots, every stage one
                       launch over all slots (no reference counterpart)
+  GalaxyArena         a GalaxyPool whose slots take their ranges from a
+                      byte-budget arena when filled (no reference counterpart)
   compute_morton_codes, radix_sort_indices, MortonTree, sort_and_build
                       src/physics/mortonTree.js:43-224 (executed on the GPU)
 
@@ -26,6 +28,12 @@ GS_E_NOMEM = -3
 GS_E_NODE_POOL = -4
 GS_E_STATE = -5
 GS_E_STACK = -6
+GS_E_FULL = -7
+
+# device bytes an arena charges per particle of a slot's (1024-aligned) range
+# and per node of its 2n + 4096 node range
+ARENA_PARTICLE_BYTES = 112
+ARENA_NODE_BYTES = 36
 
 FLAG_EXACT_FORCE = 0x1
 FLAG_TIMING = 0x2
@@ -53,6 +61,13 @@ class PoolConfig(ctypes.Structure):
                 ("bucket", ctypes.c_void_p), ("flags", ctypes.c_uint32),
                 ("device", ctypes.c_int32), ("walk_wg", ctypes.c_uint32),
                 ("radix_bits", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class ArenaConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_uint64), ("nslots", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("walk_wg", ctypes.c_uint32), ("radix_bits", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
 
 
 class Params(ctypes.Structure):
@@ -113,6 +128,9 @@ def lib():
         "gs_count_interactions": (ctypes.c_int, [_vp, ctypes.POINTER(_u64), _vp]),
         "gs_create_pool": (ctypes.c_int, [ctypes.POINTER(PoolConfig), ctypes.POINTER(_vp)]),
         "gs_pool_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
+        "gs_create_arena": (ctypes.c_int, [ctypes.POINTER(ArenaConfig), ctypes.POINTER(_vp)]),
+        "gs_arena_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u64), ctypes.POINTER(_u64),
+                                         ctypes.POINTER(_u64)]),
         "gs_slot_upload": (ctypes.c_int, [_vp, _u32, _vp, _u64]),
         "gs_slot_clear": (ctypes.c_int, [_vp, _u32]),
         "gs_slot_count": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(_u32)]),
@@ -530,6 +548,59 @@ class GalaxyPool:
         tot = _u64(0)
         self._call("gs_count_interactions", ctypes.byref(tot), None)
         return tot.value
+
+
+def arena_charge(n):
+    """Bytes an arena takes for a galaxy of n particles: its particle range
+    (n rounded up to 1024) and its node range (2n + 4096, mortonTree.js:88)."""
+    n = int(n)
+    if n <= 0:
+        return 0
+    return -(-n // 1024) * 1024 * ARENA_PARTICLE_BYTES + (2 * n + 4096) * ARENA_NODE_BYTES
+
+
+class GalaxyArena(GalaxyPool):
+    """A slot pool whose device state is sized by a byte budget: `nslots`
+    slots start empty, upload() places a galaxy in a free range of the arena
+    (raising ArenaFull, with the slot left empty, when no range holds it) and
+    clear() gives the range back.  Steps launch over the occupied slots only.
+    Every GalaxyPool call works on its slots except the stellar records."""
+
+    def __init__(self, budget_bytes, nslots, exact_force=False, timing=False, device=-1,
+                 walk_wg=0, graph=True, radix_bits=0):
+        cfg = ArenaConfig(budget_bytes=int(budget_bytes), nslots=int(nslots),
+                          flags=(FLAG_EXACT_FORCE if exact_force else 0) |
+                          (FLAG_TIMING if timing else 0) | (0 if graph else FLAG_NO_GRAPH),
+                          device=device, walk_wg=walk_wg, radix_bits=radix_bits)
+        h = ctypes.c_void_p()
+        rc = lib().gs_create_arena(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != GS_OK:
+            raise GalaxyError(rc, "gs_create_arena failed (no usable gfx950 device, a budget "
+                              "below one range, bad config or out of device memory)")
+        self._h = h
+        self.nslots = int(nslots)
+        self.budget = int(budget_bytes)
+        self.capacity = None
+        self.counts = np.zeros(self.nslots, np.int64)
+
+    def upload(self, slot, particles, **params):
+        try:
+            GalaxyPool.upload(self, slot, particles, **params)
+        except GalaxyError as e:
+            self.counts[slot] = 0
+            if e.code == GS_E_FULL:
+                raise ArenaFull(e.code, str(e)) from None
+            raise
+
+    def arena_info(self):
+        dev, used, fit = _u64(0), _u64(0), _u64(0)
+        self._call("gs_arena_info", ctypes.byref(dev), ctypes.byref(used), ctypes.byref(fit))
+        return {"device_bytes": dev.value, "used_bytes": used.value, "largest_fit": fit.value,
+                "budget_bytes": self.budget}
+
+
+class ArenaFull(GalaxyError):
+    """No free range of the arena holds the galaxy (its budget is spent)."""
 
 
 # ---- mortonTree.js surface (device-executed on host arrays) ----------------

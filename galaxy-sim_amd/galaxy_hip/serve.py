@@ -1,21 +1,25 @@
 """Continuous-batching server for independent galaxy simulations on one GPU.
 
 Jobs (a galaxy's particle records, its updateParams dictionary and a step
-count) are admitted into slots of one GalaxyPool and stepped together: every
-server step is one replay of the pool's captured step graph (rebuild +
-refresh + walk for every resident galaxy), so a job that arrives while others
-run joins the next step, and a finished job frees its slot between steps.
-Particle and tree state stay resident in HBM for the life of a job.
+count) are admitted into the slots of one GalaxyArena and stepped together:
+every server step is one replay of the arena's captured step graph (rebuild
++ refresh + walk for every resident galaxy), so a job that arrives while
+others run joins the next step, and a finished job frees its slot and its
+arena range between steps.  Particle and tree state stay resident in HBM for
+the life of a job.
 
-Admission, per waiting job in arrival order (a job that does not fit does not
-block smaller ones behind it):
-  * a free slot whose capacity holds the job (smallest such slot);
-  * HBM: the job's state footprint (state_bytes(n)) plus that of every
-    resident job stays within the budget, and the pool itself was sized
-    within the budget;
+The HBM budget is the arena: its device state is sized from the budget once,
+and a job is placed in a free range of it when admitted (arena_charge(n)
+bytes: n rounded up to 1024 particle records, 2n + 4096 node records) and
+gives the range back when it retires.  Admission, per waiting job in arrival
+order (a job that does not fit does not block smaller ones behind it):
+  * slots: a free slot (the number of galaxies resident at once);
   * latency: the resident particle total stays within `resident_limit`, so a
     step's latency stays close to its steady value instead of growing with
-    the queue (steps scale with the resident particle count).
+    the queue (steps scale with the resident particle count);
+  * budget: a free range of the arena holds the job (the arena refuses it
+    otherwise, and the job waits until retirements give ranges back).
+The summary reports how many jobs each constraint held, and the budget use.
 
 The reference runs one galaxy per NBodyGPU and has no server; the per-job
 arithmetic is the single-galaxy step, and a job's result is bit-identical to
@@ -26,37 +30,14 @@ import time
 
 import numpy as np
 
-from . import GalaxyPool
+from . import ArenaFull, GalaxyArena, arena_charge
 
-# device bytes per particle of resident state: 2 x (posm + velp) ping-pong,
-# 2 x (key + index) sort ping-pong, raw code, energy, acceleration, density,
-# interaction count; plus a node pool of 2n + 4096 32-byte records and their
-# sizes
-PARTICLE_STATE_BYTES = 2 * 32 + 2 * 8 + 4 + 4 + 16 + 4 + 4
-NODE_BYTES = 32 + 4
-
-
-def state_bytes(n):
-    return n * PARTICLE_STATE_BYTES + (2 * n + 4096) * NODE_BYTES
-
-
-def slot_classes(lo, hi, per_class):
-    """Power-of-two slot capacities covering [lo, hi]: `per_class` slots of
-    each capacity 2^k (an int or a function of the capacity)."""
-    caps = []
-    c = 1 << int(np.ceil(np.log2(lo)))
-    while True:
-        k = per_class(c) if callable(per_class) else per_class
-        caps += [c] * int(k)
-        if c >= hi:
-            break
-        c <<= 1
-    return caps
+REASONS = ("slots", "latency", "budget")
 
 
 class Job:
     __slots__ = ("id", "records", "params", "steps", "arrival", "bucket", "slot", "done",
-                 "admitted", "finished", "result", "reduce", "n")
+                 "admitted", "finished", "result", "reduce", "n", "held_by")
 
     def __init__(self, id, records, params, steps, arrival=0.0, bucket=16):
         self.id = id
@@ -72,39 +53,38 @@ class Job:
         self.finished = None
         self.result = None
         self.reduce = None
+        self.held_by = set()  # the constraints that kept it waiting at some admission
 
 
 class GalaxyServer:
-    def __init__(self, slot_caps, hbm_budget=200e9, resident_limit=None, bucket=16,
+    def __init__(self, hbm_budget=200e9, max_jobs=64, resident_limit=None, bucket=16,
                  walk_wg=0, radix_bits=0, exact_force=False, keep_results=True,
                  pool_factory=None):
-        """slot_caps: slot capacities of the pool (see slot_classes).
-        pool_factory(caps) replaces the GalaxyPool (host-side tests)."""
-        caps = sorted(int(c) for c in slot_caps)
-        pool_bytes = sum(state_bytes(c) for c in caps)
-        if pool_bytes > hbm_budget:
-            raise ValueError(f"slot pool needs {pool_bytes / 1e9:.1f} GB, over the "
-                             f"{hbm_budget / 1e9:.1f} GB budget")
+        """hbm_budget: device bytes of the arena; max_jobs: its slots (jobs
+        resident at once, at most 256); resident_limit: resident particles
+        (None: no limit).  pool_factory(budget, slots) replaces the
+        GalaxyArena (host-side tests)."""
         if pool_factory is None:
-            self.pool = GalaxyPool(caps, buckets=[bucket] * len(caps), walk_wg=walk_wg,
-                                   radix_bits=radix_bits, exact_force=exact_force)
+            self.pool = GalaxyArena(hbm_budget, max_jobs, walk_wg=walk_wg, radix_bits=radix_bits,
+                                    exact_force=exact_force)
         else:
-            self.pool = pool_factory(caps)
-        self.caps = np.array(caps, np.int64)
-        self.free = list(range(len(caps)))  # ascending capacity
-        self.hbm_budget = hbm_budget
-        self.pool_bytes = pool_bytes
-        self.resident_limit = resident_limit or int(self.caps.sum())
+            self.pool = pool_factory(hbm_budget, max_jobs)
+        self.hbm_budget = int(hbm_budget)
+        self.max_jobs = int(max_jobs)
+        self.free = list(range(self.max_jobs))
+        self.resident_limit = resident_limit
         self.keep_results = keep_results
         self.waiting = collections.deque()
         self.active = {}
         self.finished = []
         self.resident_n = 0
-        self.resident_bytes = 0
+        self.resident_bytes = 0  # sum of arena_charge over resident jobs
+        self.refusals = dict.fromkeys(REASONS, 0)  # admission attempts refused, per reason
         self.step_lat = []
         self.step_particles = []
         self.step_jobs = []
         self.step_queued = []  # jobs waiting for admission when the step ran
+        self.step_bytes = []   # arena bytes taken by the resident jobs when the step ran
         self.step_end = []
         self.particle_steps = 0
 
@@ -119,38 +99,42 @@ class GalaxyServer:
 
     # -- admission ---------------------------------------------------------
     def submit(self, job):
-        if job.n > self.caps.max():
-            raise ValueError(f"job {job.id}: n={job.n} exceeds the largest slot")
         self.waiting.append(job)
 
-    def _fits(self, job):
-        if self.resident_n + job.n > self.resident_limit and self.active:
-            return -1
-        if self.resident_bytes + state_bytes(job.n) > self.hbm_budget:
-            return -1
-        for i, s in enumerate(self.free):
-            if self.caps[s] >= job.n:
-                return i
-        return -1
+    def _hold(self, job, reason):
+        self.refusals[reason] += 1
+        job.held_by.add(reason)
 
     def admit(self, now=None):
-        """Move waiting jobs into free slots; returns the number admitted."""
+        """Move waiting jobs into free slots and arena ranges; returns the
+        number admitted."""
         admitted = 0
         keep = collections.deque()
         while self.waiting:
             job = self.waiting.popleft()
-            i = self._fits(job)
-            if i < 0:
+            if not self.free:
+                self._hold(job, "slots")
                 keep.append(job)
                 continue
-            slot = self.free.pop(i)
-            self.pool.upload(slot, job.records, **job.params)
+            if (self.resident_limit is not None and self.active
+                    and self.resident_n + job.n > self.resident_limit):
+                self._hold(job, "latency")
+                keep.append(job)
+                continue
+            slot = self.free[0]
+            try:
+                self.pool.upload(slot, job.records, **job.params)
+            except ArenaFull:
+                self._hold(job, "budget")
+                keep.append(job)
+                continue
+            self.free.pop(0)
             self.pool.set_tree(slot, job.bucket)
             job.slot = slot
             job.admitted = time.perf_counter() if now is None else now
             self.active[slot] = job
             self.resident_n += job.n
-            self.resident_bytes += state_bytes(job.n)
+            self.resident_bytes += arena_charge(job.n)
             admitted += 1
         self.waiting = keep
         return admitted
@@ -168,6 +152,7 @@ class GalaxyServer:
         self.step_particles.append(self.resident_n)
         self.step_jobs.append(len(self.active))
         self.step_queued.append(len(self.waiting))
+        self.step_bytes.append(self.resident_bytes)
         self.step_end.append(time.perf_counter())
         self.particle_steps += self.resident_n
         done = []
@@ -184,12 +169,12 @@ class GalaxyServer:
         job.reduce = self.pool.reduce(slot)  # FP64 energy / momentum of the last step
         if self.keep_results:
             job.result = self.pool.particles(slot)
-        self.pool.clear(slot)
+        self.pool.clear(slot)  # the slot's arena ranges go back
         del self.active[slot]
         self.free.append(slot)
-        self.free.sort(key=lambda s: (self.caps[s], s))
+        self.free.sort()
         self.resident_n -= job.n
-        self.resident_bytes -= state_bytes(job.n)
+        self.resident_bytes -= arena_charge(job.n)
         job.finished = time.perf_counter()
         self.finished.append(job)
 
@@ -213,8 +198,11 @@ class GalaxyServer:
                 self.step()
                 if log and len(self.step_lat) % 100 == 0:
                     log(f"server step {len(self.step_lat)}: resident {len(self.active)} jobs / "
-                        f"{self.resident_n} particles, waiting {len(self.waiting)}, "
-                        f"finished {len(self.finished)}")
+                        f"{self.resident_n} particles / {self.resident_bytes / 1e9:.2f} GB, "
+                        f"waiting {len(self.waiting)}, finished {len(self.finished)}")
+            elif self.waiting:
+                raise RuntimeError(f"{len(self.waiting)} jobs can never be admitted (each "
+                                   "needs more than the whole budget)")
             elif pending:
                 time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
         t_end = time.perf_counter()
@@ -238,6 +226,8 @@ class GalaxyServer:
         jlat = np.repeat(lat, np.array(self.step_jobs, np.int64)) if lat.size else lat
         jl = np.array([j.finished - j.arrival for j in self.finished]) if self.finished else []
         wait = np.array([j.admitted - j.arrival for j in self.finished]) if self.finished else []
+        sb = np.array(self.step_bytes, np.float64)
+        info = self.pool.arena_info() if hasattr(self.pool, "arena_info") else {}
         return {
             "jobs": len(self.finished),
             "steps": len(self.step_lat),
@@ -257,7 +247,15 @@ class GalaxyServer:
             "job_latency_p99_s": float(np.percentile(jl, 99)) if len(jl) else 0.0,
             "queue_wait_p50_s": float(np.percentile(wait, 50)) if len(wait) else 0.0,
             "queue_wait_p99_s": float(np.percentile(wait, 99)) if len(wait) else 0.0,
-            "pool_bytes": self.pool_bytes,
             "hbm_budget": self.hbm_budget,
-            "slots": int(self.caps.size),
+            "arena_device_bytes": info.get("device_bytes"),
+            "budget_used_peak_bytes": float(sb.max()) if sb.size else 0.0,
+            "budget_used_mean_bytes": float(sb.mean()) if sb.size else 0.0,
+            "budget_used_peak_frac": float(sb.max()) / self.hbm_budget if sb.size else 0.0,
+            "slots": self.max_jobs,
+            "resident_limit": self.resident_limit,
+            # jobs that some admission found held by each constraint, and the
+            # refused admission attempts per constraint
+            "jobs_held_by": {r: sum(r in j.held_by for j in self.finished) for r in REASONS},
+            "refusals": dict(self.refusals),
         }

@@ -45,6 +45,7 @@ extern "C" {
 #define GS_E_NODE_POOL (-4) /* tree needs more nodes than the pool holds */
 #define GS_E_STATE (-5)     /* call order: e.g. force before any sort */
 #define GS_E_STACK (-6)     /* walk stack overflow (unreachable, depth <= 10) */
+#define GS_E_FULL (-7)      /* arena pool: no free range holds the galaxy */
 
 /* gs_config.flags */
 #define GS_FLAG_EXACT_FORCE 0x1u /* IEEE sqrt and '/' in every interaction:   \
@@ -242,6 +243,39 @@ typedef struct gs_pool_config {
 int gs_create_pool(const gs_pool_config* cfg, gs_sim** out);
 /* Slot count and device bytes held by the pool. */
 int gs_pool_info(gs_sim* sim, uint32_t* nslots, uint64_t* device_bytes);
+
+/* Arena pools: a pool whose device state is sized by a byte budget instead
+ * of fixed slot capacities.  No counterpart in the reference (a serving
+ * loop's admission control).  Slots start empty; gs_slot_upload of n > 0
+ * records takes a particle range of align_up(n, 1024) and a node range of
+ * 2n + 4096 (mortonTree.js:88) from the arena's free lists (best fit), and
+ * GS_E_FULL when none holds them (the budget is spent: the caller queues the
+ * galaxy); n = 0 or gs_slot_clear gives the ranges back.  Launches cover the
+ * occupied slots only (their tables are rebuilt, and the step graphs
+ * captured again, before the first launch after a change), so a step costs
+ * what the resident galaxies need whatever the budget.  Every slot call
+ * above works on an arena slot (gs_slot_reduce until the next launch after
+ * a slot changes); gs_upload_tree, gs_apply_permutation, stellar records
+ * and the mortonTree.js calls below need a plain handle (GS_E_INVALID). */
+#define GS_ARENA_PARTICLE_BYTES 112u /* per particle of a slot's range */
+#define GS_ARENA_NODE_BYTES 36u      /* per node of a slot's range */
+typedef struct gs_arena_config {
+  uint64_t budget_bytes; /* device bytes the whole arena may hold */
+  uint32_t nslots;       /* 1 .. 256 slots (galaxies resident at once) */
+  uint32_t flags;        /* GS_FLAG_* */
+  int32_t device;        /* HIP device ordinal, -1 = current device */
+  uint32_t walk_wg;      /* as gs_config.walk_wg */
+  uint32_t radix_bits;   /* as gs_config.radix_bits */
+  uint32_t reserved[5];
+} gs_arena_config;
+
+int gs_create_arena(const gs_arena_config* cfg, gs_sim** out);
+/* device_bytes: held by the arena (<= the budget); used_bytes: taken by the
+ * occupied slots' ranges (GS_ARENA_PARTICLE_BYTES per particle of each
+ * range, GS_ARENA_NODE_BYTES per node); largest_fit: the most particles one
+ * more galaxy may have now.  Any output may be NULL. */
+int gs_arena_info(gs_sim* sim, uint64_t* device_bytes, uint64_t* used_bytes,
+                  uint64_t* largest_fit);
 /* gs_upload into one slot: n (0 .. capacity) records; n = 0 empties it. */
 int gs_slot_upload(gs_sim* sim, uint32_t slot, const float* particles,
                    uint64_t n);

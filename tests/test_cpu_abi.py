@@ -63,6 +63,18 @@ def test_quantize_matches_native_double(tmp_path):
     assert "bad=0" in r.stdout
 
 
+def test_arena_free_lists_sanitized(tmp_path):
+    """The arena's free lists (csrc/arena.h) against a unit bitmap over random
+    take / give sequences, built with AddressSanitizer and UBSan."""
+    exe = tmp_path / "arena_check"
+    subprocess.run(["g++", "-O1", "-g", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "native", "arena_check.cpp")], check=True)
+    r = subprocess.run([str(exe), "1500"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0" in r.stdout
+
+
 def test_initial_conditions_deterministic():
     from galaxy_hip import ic
     a = ic.make("cpu_disk_16k")

@@ -1,12 +1,13 @@
 """The continuous-batching server on the GPU: jobs that arrive while others
-run are admitted as slots free up, and each job's result is the same alone
-and in the mix (bit-identical: a slot runs the single-galaxy arithmetic on
-slot-local indices)."""
+run are admitted as slots and arena ranges free up, and each job's result is
+the same alone and in the mix (bit-identical: a slot runs the single-galaxy
+arithmetic on slot-local indices).  In the second test the HBM budget, not
+the slots, holds jobs back."""
 import numpy as np
 import pytest
 
 import galaxy_hip
-from galaxy_hip import ic, serve
+from galaxy_hip import arena_charge, ic, serve
 from gpu_util import banner
 
 pytestmark = pytest.mark.gpu
@@ -23,17 +24,7 @@ def _jobs():
     return out
 
 
-@pytest.mark.parametrize("exact", [False, True])
-def test_served_jobs_equal_alone(exact):
-    banner(f"served vs alone exact={exact}")
-    specs = _jobs()
-    caps = serve.slot_classes(2048, 65536, 1)  # 6 slots for 9 jobs: some must wait
-    with serve.GalaxyServer(caps, resident_limit=60000, exact_force=exact) as srv:
-        jobs = [serve.Job(j["id"], r, PARAMS, j["steps"], 0.0) for j, r in specs]
-        summ = srv.run(jobs, realtime=False)
-    assert summ["jobs"] == len(specs)
-    assert any(j.admitted > min(k.finished for k in jobs) for j in jobs), \
-        "no job entered while others ran"
+def _check_alone(jobs, specs, exact):
     for job, (spec, rec) in zip(jobs, specs):
         banner(f"  job {spec['id']} n={spec['n']} {spec['preset']} steps={spec['steps']}")
         sim = galaxy_hip.NBodyGPU(rec, bucket=16, exact_force=exact)
@@ -47,3 +38,34 @@ def test_served_jobs_equal_alone(exact):
         assert np.array_equal(job.result.view(np.uint32), want.view(np.uint32))
         for k in red:
             assert job.reduce[k] == pytest.approx(red[k], rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_served_jobs_equal_alone(exact):
+    banner(f"served vs alone exact={exact}")
+    specs = _jobs()
+    # 6 slots for 9 jobs: some must wait
+    with serve.GalaxyServer(hbm_budget=2e9, max_jobs=6, resident_limit=60000,
+                            exact_force=exact) as srv:
+        jobs = [serve.Job(j["id"], r, PARAMS, j["steps"], 0.0) for j, r in specs]
+        summ = srv.run(jobs, realtime=False)
+    assert summ["jobs"] == len(specs)
+    assert any(j.admitted > min(k.finished for k in jobs) for j in jobs), \
+        "no job entered while others ran"
+    _check_alone(jobs, specs, exact)
+
+
+def test_budget_holds_jobs_and_results_equal_alone():
+    banner("served under a tight HBM budget")
+    specs = _jobs()
+    big = max(j["n"] for j, _ in specs)
+    budget = int(2.5 * arena_charge(big))
+    with serve.GalaxyServer(hbm_budget=budget, max_jobs=9) as srv:
+        jobs = [serve.Job(j["id"], r, PARAMS, j["steps"], 0.0) for j, r in specs]
+        summ = srv.run(jobs, realtime=False)
+    banner(f"  budget {budget} B: peak use {summ['budget_used_peak_bytes']:.0f} B, held "
+           f"{summ['jobs_held_by']}")
+    assert summ["jobs"] == len(specs)
+    assert summ["jobs_held_by"]["budget"] > 0 and summ["jobs_held_by"]["slots"] == 0
+    assert summ["budget_used_peak_bytes"] <= budget
+    _check_alone(jobs, specs, False)

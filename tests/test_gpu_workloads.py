@@ -8,8 +8,9 @@ force path's 1e-5 bar on every particle of the 500,000-particle disk.
   the oracle's steps bit for bit.
 * Served (Poisson job stream, N log-uniform in [8192, 2^20], disk / Plummer /
   collision): eleven jobs of the stream's recipe, the two largest near 2^20
-  particles, served through fewer slots than jobs (so jobs enter while others
-  run) with the benchmark's bucket 24; each result equals the job run alone.
+  particles, served from an arena with fewer slots than jobs (so jobs enter
+  while others run) with the benchmark's bucket 24; each result equals the
+  job run alone.
 * The default path's forces on all 500,000 particles of the gas disk against
   the transliterated walk, to the bar of gpu_util.check_force_bar (the
   headline's 4,194,304 are checked in test_gpu_full.py)."""
@@ -99,9 +100,10 @@ def test_served_stream_jobs_equal_alone():
     assert {j["preset"] for j in specs} == {"disk", "plummer", "collision"}
     params = dict(dt=0.005, G=1.0, epsilon=0.01, theta=0.5)
     bucket = TUNED["bucket"]
-    caps = serve.slot_classes(8192, 1 << 20, 1)  # 8 slots for 11 jobs: some must wait
     recs = [ic.job_records(j) for j in specs]
-    with serve.GalaxyServer(caps, resident_limit=3 << 20, bucket=bucket) as srv:
+    # 8 slots for 11 jobs: some must wait
+    with serve.GalaxyServer(hbm_budget=4e9, max_jobs=8, resident_limit=3 << 20,
+                            bucket=bucket) as srv:
         jobs = [serve.Job(j["id"], r, params, 2 + j["id"] % 3, 0.0, bucket=bucket)
                 for j, r in zip(specs, recs)]
         summ = srv.run(jobs, realtime=False)
