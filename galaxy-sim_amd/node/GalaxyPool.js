@@ -25,8 +25,14 @@ export class GalaxyPool {
    * capacities: particles per slot (Array or Uint32Array, 1 .. 256 slots).
    * options: {buckets, exactForce, device, walkWg, radixBits, graph}.
    */
-  constructor(device, capacities, options) {
+  constructor(device, capacities, options, arena) {
     const o = options || {};
+    if (arena) {  // GalaxyArena: slots without fixed capacities
+      this.capacities = new Array(arena.slots).fill(0);
+      this.counts = new Array(arena.slots).fill(0);
+      this.h = arena.create();
+      return;
+    }
     this.capacities = Array.from(capacities);
     this.counts = new Array(this.capacities.length).fill(0);
     this.h = addon.createPool({
@@ -79,4 +85,43 @@ export class GalaxyPool {
     if (this.h) addon.destroy(this.h);
     this.h = null;
   }
+}
+
+/**
+ * A pool whose device state is sized by a byte budget (gs_create_arena):
+ * `slots` slots start empty; upload() places a galaxy in a free range of the
+ * arena and throws an Error with code 'GS_E_FULL' (the slot left empty) when
+ * no range holds it; clear() gives the range back.  Steps launch over the
+ * occupied slots only.  No stellar records.
+ */
+export class GalaxyArena extends GalaxyPool {
+  /** options: {exactForce, device, walkWg, radixBits, graph}. */
+  constructor(device, budgetBytes, slots, options) {
+    const o = options || {};
+    super(null, [], null, {
+      create: () => addon.createArena({
+        budgetBytes,
+        slots,
+        exactForce: o.exactForce ? 1 : 0,
+        device: device && typeof device.ordinal === 'number' ? device.ordinal : -1,
+        walkWg: o.walkWg || 0,
+        radixBits: o.radixBits || 0,
+        graph: o.graph === false ? 0 : 1,
+      }),
+      slots,
+    });
+    this.budgetBytes = budgetBytes;
+  }
+
+  upload(slot, particleData, params) {
+    try {
+      super.upload(slot, particleData, params);
+    } catch (e) {
+      this.counts[slot] = 0;
+      throw e;
+    }
+  }
+
+  /** {deviceBytes, usedBytes, largestFit}: the arena, its resident ranges, the largest galaxy that fits now. */
+  arenaInfo() { return addon.arenaInfo(this.h); }
 }

@@ -22,10 +22,25 @@ namespace {
     }                                                     \
   } while (0)
 
+// the thrown Error's code names the status (err.code === 'GS_E_FULL' when an
+// arena has no range for a galaxy)
+const char* gs_code_name(int rc) {
+  switch (rc) {
+    case GS_E_INVALID: return "GS_E_INVALID";
+    case GS_E_DEVICE: return "GS_E_DEVICE";
+    case GS_E_NOMEM: return "GS_E_NOMEM";
+    case GS_E_NODE_POOL: return "GS_E_NODE_POOL";
+    case GS_E_STATE: return "GS_E_STATE";
+    case GS_E_STACK: return "GS_E_STACK";
+    case GS_E_FULL: return "GS_E_FULL";
+    default: return "GS_E_UNKNOWN";
+  }
+}
+
 napi_value throw_gs(napi_env env, gs_sim* s, int rc, const char* what) {
   std::string msg = std::string(what) + " failed (" + std::to_string(rc) + "): " +
                     (s ? gs_last_error(s) : "no handle");
-  napi_throw_error(env, nullptr, msg.c_str());
+  napi_throw_error(env, gs_code_name(rc), msg.c_str());
   return nullptr;
 }
 
@@ -676,6 +691,48 @@ napi_value PoolInfo(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// createArena({budgetBytes, slots, exactForce, device, walkWg, radixBits, graph}) -> handle
+napi_value CreateArena(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 1) {
+    napi_throw_type_error(env, nullptr, "createArena(options)");
+    return nullptr;
+  }
+  napi_value o = a.v[0];
+  gs_arena_config cfg;
+  memset(&cfg, 0, sizeof cfg);
+  cfg.budget_bytes = (uint64_t)num_prop(env, o, "budgetBytes", 0);
+  cfg.nslots = (uint32_t)num_prop(env, o, "slots", 0);
+  cfg.flags = (num_prop(env, o, "exactForce", 0) != 0 ? GS_FLAG_EXACT_FORCE : 0u) |
+              (num_prop(env, o, "graph", 1) == 0 ? GS_FLAG_NO_GRAPH : 0u);
+  cfg.device = (int32_t)num_prop(env, o, "device", -1);
+  cfg.walk_wg = (uint32_t)num_prop(env, o, "walkWg", 0);
+  cfg.radix_bits = (uint32_t)num_prop(env, o, "radixBits", 0);
+  gs_sim* s = nullptr;
+  int rc = gs_create_arena(&cfg, &s);
+  if (rc != GS_OK) return throw_gs(env, nullptr, rc, "gs_create_arena");
+  Holder* h = new Holder{s};
+  napi_value ext;
+  CALL(env, napi_create_external(env, h, finalize_holder, nullptr, &ext));
+  return ext;
+}
+
+// arenaInfo(h) -> {deviceBytes, usedBytes, largestFit}
+napi_value ArenaInfo(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  SIM_OR_RETURN(s, a.v[0]);
+  uint64_t dev = 0, used = 0, fit = 0;
+  int rc = gs_arena_info(s, &dev, &used, &fit);
+  if (rc) return throw_gs(env, s, rc, "gs_arena_info");
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "deviceBytes", num(env, (double)dev));
+  napi_set_named_property(env, o, "usedBytes", num(env, (double)used));
+  napi_set_named_property(env, o, "largestFit", num(env, (double)fit));
+  return o;
+}
+
 // uploadStellar(h, Float32Array(4 n)) / slotUploadStellar(h, slot, Float32Array)
 napi_value UploadStellar(napi_env env, napi_callback_info info) {
   Args a;
@@ -808,6 +865,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"computeMortonCodes", ComputeMortonCodes}, {"radixSortIndices", RadixSortIndices},
       {"treeBuild", TreeBuild}, {"version", Version}, {"deviceCount", DeviceCount},
       {"createPool", CreatePool}, {"poolInfo", PoolInfo}, {"slotUpload", SlotUpload},
+      {"createArena", CreateArena}, {"arenaInfo", ArenaInfo},
       {"slotClear", SlotClear}, {"slotCount", SlotCount}, {"slotSetParams", SlotSetParams},
       {"slotSetTree", SlotSetTree}, {"slotDownloadParticles", SlotDownloadParticles},
       {"slotDownloadEnergy", SlotDownloadEnergy}, {"slotReduce", SlotReduce},

@@ -32,6 +32,17 @@ def test_node_pool_parity():
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed on this machine")
+def test_node_arena_parity():
+    """GalaxyPool.js GalaxyArena: placed, removed and re-placed galaxies equal
+    NBodyGPU.js run alone, bit for bit; a full arena throws GS_E_FULL."""
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "node", "arena_parity.mjs")],
+                       capture_output=True, text=True, timeout=240, cwd=ROOT)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "node arena parity ok" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed on this machine")
 def test_node_stellar_flow():
     """stellarBuffer follows applyPermutation; encodeStarFormation forms stars."""
     r = subprocess.run(["node", os.path.join(ROOT, "tests", "node", "stellar_flow.mjs")],

@@ -87,3 +87,55 @@ def test_refresh_f32_within_rounding_of_fp64():
     size = f64[:, 4].view(np.float32).astype(np.float64)
     # a COM lies inside its node; f32 sums err by far less than 1e-4 of its size
     assert (np.abs(c32 - c64).max(1) <= 1e-4 * size + 1e-6).all()
+
+
+def test_direct_sum_closed_forms():
+    """oracle.direct_sum_f64 (the FP64 reference of the force-error figures)
+    against closed forms: the softened two-body force G m r / (r^2 + eps^2)^1.5,
+    zero net force at the centre of a cube of equal masses, and Newton's third
+    law (sum of m a over a random set vanishes)."""
+    eps = 0.1
+    two = np.zeros((2, 8), np.float32)
+    two[0, :4] = (0.0, 0.0, 0.0, 2.0)
+    two[1, :4] = (1.5, -0.5, 0.25, 3.0)
+    acc = oracle.direct_sum_f64(two.reshape(-1), 0.7, eps * eps, np.array([0, 1], np.uint32))
+    r = two[1, :3].astype(np.float64) - two[0, :3]
+    inv3 = (r @ r + eps * eps) ** -1.5
+    np.testing.assert_allclose(acc[0], 0.7 * 3.0 * r * inv3, rtol=1e-12)
+    np.testing.assert_allclose(acc[1], -0.7 * 2.0 * r * inv3, rtol=1e-12)
+    cube = np.zeros((9, 8), np.float32)
+    k = 0
+    for x in (-1, 1):
+        for y in (-1, 1):
+            for z in (-1, 1):
+                cube[k, :4] = (x, y, z, 1.0)
+                k += 1
+    cube[8, :4] = (0, 0, 0, 5.0)
+    a = oracle.direct_sum_f64(cube.reshape(-1), 1.0, 0.0, np.array([8], np.uint32))
+    assert np.abs(a).max() < 1e-14
+    rng = np.random.default_rng(3)
+    pts = np.zeros((300, 8), np.float32)
+    pts[:, :3] = rng.normal(size=(300, 3))
+    pts[:, 3] = rng.uniform(0.5, 2.0, 300)
+    a = oracle.direct_sum_f64(pts.reshape(-1), 1.0, 0.01, np.arange(300, dtype=np.uint32))
+    m = pts[:, 3].astype(np.float64)
+    net = (m[:, None] * a).sum(0)
+    assert np.abs(net).max() < 1e-10 * np.abs(m[:, None] * a).sum()
+
+
+def test_walk_at_theta_zero_is_the_direct_sum():
+    """At theta = 0 no node passes s^2 < theta^2 d^2 (and none passes
+    s^2 < eps^2/4 at this eps), so the oracle's walk opens the whole tree: every
+    particle interacts with each of the other N - 1 exactly once, and its force
+    is the FP64 direct sum to f32 accumulation error."""
+    data = ic.make("cpu_disk_16k", n=1500)
+    n = data.shape[0]
+    ref = oracle.sort_and_build(data.reshape(-1), n, bucket=8)
+    op = oracle.pack_params(n, dt=0.005, G=1.0, epsilon=0.05, theta=0.0)
+    _, _, acc4, _, inter = oracle.walk(ref["sorted"], ref["nodes"], op)
+    assert (inter == n - 1).all()
+    ds = oracle.direct_sum_f64(ref["sorted"], 1.0, float(np.float32(0.05 * 0.05)),
+                               np.arange(n, dtype=np.uint32))
+    a = acc4[:, :3].astype(np.float64)
+    err = np.linalg.norm(a - ds, axis=1) / np.linalg.norm(ds, axis=1)
+    assert err.max() < 2e-5, err.max()
