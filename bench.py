@@ -11,8 +11,9 @@ step.  Prints ONE JSON line.
 A step is gs_step(sim, 1, 1): bbox + Morton encode, 4 radix passes (last one
 gathers the particle records), 11 BFS build levels, FP64 refresh, walk with
 halos + kick-drift + FP64 reductions.  Inputs are resident in HBM before the
-timed region; every step is bracketed by a stream synchronisation so the
-per-step latencies give p50/p99.  Each timed step replays two captured graphs
+timed region.  Throughput (value, ms_per_step) is K steps launched back to
+back with a synchronisation at each end; K more steps, each bracketed by a
+stream synchronisation, give the p50/p99 step latency.  Each step replays two captured graphs
 (rebuild + refresh, then the walk) with HIP events the library records on its
 own stream between them, which time the walk kernel over the timed region;
 a few more steps with direct launches give the per-stage breakdown.
@@ -155,8 +156,10 @@ def bench_config(name, steps, cpu_steps, fe_sample, args):
         sim.step(args.warmup, sort_every=1)
         sim.sync()
         t0 = time.perf_counter()
-        lat = _timed_steps(sim, steps)
+        sim.step(steps, sort_every=1)  # back to back
+        sim.sync()
         total = time.perf_counter() - t0
+        lat = _timed_steps(sim, steps)  # each step synchronised: latency
         inter = sim.count_interactions()
     finally:
         sim.destroy()
@@ -233,8 +236,10 @@ def bench_batch(args):
         pool.step(args.warmup, sort_every=1)
         pool.sync()
         t0 = time.perf_counter()
-        lat = _timed_steps(pool, args.batch_steps)
+        pool.step(args.batch_steps, sort_every=1)  # back to back
+        pool.sync()
         t_pool = time.perf_counter() - t0
+        lat = _timed_steps(pool, args.batch_steps)  # each step synchronised: latency
         inter = pool.count_interactions()
     finally:
         pool.destroy()
@@ -642,15 +647,16 @@ def bench_headline(args):
     sim.sync()
     sim.stage_times(reset=True)
 
-    lat = []
+    # throughput: K steps launched back to back (the simulation loop), one
+    # synchronisation at each end
     t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        sim.step(1, sort_every=1)
-        sim.sync()
-        lat.append(time.perf_counter() - t0)
+    sim.step(args.steps, sort_every=1)
+    sim.sync()
     total = time.perf_counter() - t_start
     stages = sim.stage_times(reset=True)
+    # latency: K more steps, each followed by a synchronisation (p50/p99)
+    lat = _timed_steps(sim, args.steps)
+    sim.stage_times(reset=True)
     info = sim.tree_info()
     inter = sim.count_interactions()
     sim.destroy()
@@ -719,6 +725,9 @@ def bench_headline(args):
         "stages_ms": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in direct.items()},
         "stages_note": "stages_ms: 5 steps with direct launches and an event pair per stage; "
                        "the timed steps replay captured graphs",
+        "timing_note": "value and ms_per_step: K steps launched back to back with one "
+                       "synchronisation at each end; p50_ms / p99_ms: K more steps, each "
+                       "followed by a stream synchronisation",
         "interactions_per_s": inter / (ms_per_step * 1e-3),
         "cpu_baseline": None,
     }

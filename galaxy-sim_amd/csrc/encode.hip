@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
   const uint32_t slot = blockIdx.x;
   const DevSlot sl = slots[slot];
   DevScalars* sc = scs + slot;
+  __shared__ uint64_t s_size;
   for (int k = threadIdx.x; k < GS_HIST_WORDS; k += blockDim.x) sc->sort_hist[k] = 0;
   float r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   uint32_t c = 0;
@@ -183,17 +184,7 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
       sc->s32 = __uint_as_float(sd::to_f32(sc->s_bits));
     }
     sc->size_bits = size;
-    // node sizes per depth: f32(rootSize * 0.5^d) (mortonTree.js:124,161)
-    uint64_t sz = size;
-    float min_s2 = INFINITY;
-    for (int dpt = 0; dpt <= GS_MAX_LEVELS; dpt++) {
-      const float f = __uint_as_float(sd::to_f32(sz));
-      sc->level_size[dpt] = f;
-      // nodes exist at depths 0..GS_MAX_LEVEL only; fminf drops a NaN
-      if (dpt <= GS_MAX_LEVEL) min_s2 = fminf(min_s2, __int_as_float(gs_s2_bits(f)));
-      if (((sz >> 52) & 0x7ff) != 0x7ff) sz = sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
-    }
-    sc->min_s2 = min_s2;
+    s_size = size;
     // level 0 holds the root alone (none for an empty slot)
     for (int k = 0; k <= GS_MAX_LEVELS; k++) {
       sc->level_start[k] = 0;
@@ -204,11 +195,31 @@ __global__ __launch_bounds__(256) void root_kernel(const float* __restrict__ par
     sc->num_levels = 0;
   }
   __syncthreads();
-  // opening thresholds of each depth's size for the slot's theta^2, one lane
-  // per depth (their bisections run side by side)
-  if (threadIdx.x <= GS_MAX_LEVELS)
-    sc->level_thr[threadIdx.x] =
-        gs_open_threshold(__int_as_float(gs_s2_bits(sc->level_size[threadIdx.x])), prm[slot].theta2);
+  // node sizes per depth, f32(rootSize * 0.5^d) (mortonTree.js:124,161), and
+  // their opening thresholds for the slot's theta^2, one lane per depth.  A
+  // halving of a binary64 whose exponent field is 2 .. 0x7fe is exact and
+  // lowers the field by one; Inf and NaN stay; below that the soft-double
+  // multiply rounds as the JS multiplication does.
+  float s2 = INFINITY;  // size^2 of this lane's depth (lanes past the depths: +Inf)
+  if (threadIdx.x <= GS_MAX_LEVELS) {
+    uint64_t sz = s_size;
+    for (uint32_t k = 0; k < threadIdx.x; k++) {
+      const uint32_t e = (uint32_t)(sz >> 52) & 0x7ffu;
+      if (e == 0x7ffu) break;
+      sz = e >= 2u ? sz - (1ull << 52) : sd::mul(sz, 0x3FE0000000000000ull /* 0.5 */);
+    }
+    const float f = __uint_as_float(sd::to_f32(sz));
+    s2 = __int_as_float(gs_s2_bits(f));
+    sc->level_size[threadIdx.x] = f;
+    sc->level_thr[threadIdx.x] = gs_open_threshold(s2, prm[slot].theta2);
+  }
+  // smallest size^2 over the depths that hold nodes (0 .. GS_MAX_LEVEL);
+  // fminf drops a NaN
+  if (threadIdx.x < 64) {
+    float v = threadIdx.x <= GS_MAX_LEVEL ? s2 : INFINITY;
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    if (threadIdx.x == 0) sc->min_s2 = v;
+  }
   __syncthreads();
   // level 0 = the root, covering [0, n); an empty slot has no tree
   if (threadIdx.x == 0 && sl.n) {
