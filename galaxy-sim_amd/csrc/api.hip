@@ -11,6 +11,7 @@
 #include <string>
 
 #include "gs_internal.h"
+#include "tree_check.h"
 
 #define GS_VERSION "galaxy_hip 0.2 (gfx950)"
 
@@ -961,11 +962,8 @@ int gs_apply_permutation(gs_sim* s, const uint32_t* perm, uint64_t n) {
     need(!s->arena, GS_E_INVALID, "applyPermutation needs a plain handle, not an arena");
     const HostSlot& h = s->hs[0];
     need(perm != nullptr && n == h.d.n, GS_E_INVALID, "permutation needs N entries");
-    std::vector<uint8_t> seen(n, 0);
-    for (uint64_t i = 0; i < n; i++) {
-      need(perm[i] < n && !seen[perm[i]], GS_E_INVALID, "not a permutation of [0, N)");
-      seen[perm[i]] = 1;
-    }
+    const char* why = gs::check_permutation(perm, n);
+    need(why == nullptr, GS_E_INVALID, why ? why : "");
     uint32_t* dperm = nullptr;
     dalloc(&dperm, n, "perm");
     hipError_t e = hipMemcpyAsync(dperm, perm, 4 * n, hipMemcpyHostToDevice, s->stream);
@@ -990,49 +988,12 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
          "bad level ranges");
     // host validation: every index the device will follow stays in bounds;
     // the walk's (skip, depth) links follow from the BFS numbering
-    std::vector<int2> links(num_nodes, make_int2(-1, 0));
+    std::vector<gs::TreeLink> links;
     float min_s2 = INFINITY;  // the walk's eps^2/4 opening test needs it
     uint32_t massless = 0;    // and its massless-node test this
-    for (uint64_t i = 0; i < num_nodes; i++) {
-      const uint32_t* w = nodes + i * 16;
-      float size, mass;
-      memcpy(&size, &w[4], 4);
-      memcpy(&mass, &w[3], 4);
-      min_s2 = fminf(min_s2, host_s2(size));
-      if (mass == 0.0f) massless = 1;
-      if (w[5] == 1u) {
-        need(w[6] <= h.d.n && w[7] <= h.d.n - w[6], GS_E_INVALID, "leaf range outside [0, N)");
-      } else {
-        int32_t prev = -1;
-        for (int k = 0; k < 8; k++) {
-          const int32_t c = (int32_t)w[8 + k];
-          need(c < 0 || ((uint64_t)c > i && (uint64_t)c < num_nodes), GS_E_INVALID,
-               "child index must follow its parent and lie inside the tree");
-          if (c >= 0) {
-            need(links[c].y == 0, GS_E_INVALID, "node has two parents");
-            links[c] = make_int2(prev >= 0 ? prev : links[i].x, links[i].y + 1);
-            need(links[c].y <= GS_MAX_LEVEL, GS_E_INVALID, "tree deeper than 10 levels");
-            prev = c;
-          }
-        }
-      }
-    }
-    // levelRanges (mortonTree.js:158): [start, count] per depth, ascending
-    // and contiguous from node 0, covering every node, each node at the depth
-    // of its range; the refresh walks these ranges without further bounds
-    if (num_levels > 0) {
-      uint64_t next = 0;
-      for (uint32_t k = 0; k < num_levels; k++) {
-        const uint64_t start = level_ranges[2 * k], count = level_ranges[2 * k + 1];
-        need(start == next && count <= num_nodes - start, GS_E_INVALID,
-             "level ranges must be contiguous from node 0 and inside the tree");
-        for (uint64_t i = start; i < start + count; i++)
-          need((uint32_t)links[i].y == k, GS_E_INVALID,
-               "a node's depth differs from the level range holding it");
-        next = start + count;
-      }
-      need(next == num_nodes, GS_E_INVALID, "level ranges must cover every node");
-    }
+    const char* why = gs::check_tree64(nodes, num_nodes, h.d.n, level_ranges, num_levels,
+                                       GS_MAX_LEVEL, links, min_s2, massless);
+    need(why == nullptr, GS_E_INVALID, why ? why : "");
     ensure_node64(s, s->max_ncap);
     check(hipMemcpyAsync(s->node64, nodes, num_nodes * 64, hipMemcpyHostToDevice, s->stream),
           "nodes");
@@ -1046,6 +1007,7 @@ int gs_upload_tree(gs_sim* s, const uint32_t* nodes, uint64_t num_nodes,
     }
     check(hipMemcpyAsync(s->sc, &sc, sizeof sc, hipMemcpyHostToDevice, s->stream), "scalars");
     check(hipMemcpyAsync(s->massless, &massless, 4, hipMemcpyHostToDevice, s->stream), "flag");
+    static_assert(sizeof(gs::TreeLink) == sizeof(int2), "links are copied as int2 (skip, depth)");
     int2* dlinks = nullptr;
     dalloc(&dlinks, num_nodes, "links");
     hipError_t e = hipMemcpyAsync(dlinks, links.data(), sizeof(int2) * num_nodes,

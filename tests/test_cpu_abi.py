@@ -75,6 +75,36 @@ def test_arena_free_lists_sanitized(tmp_path):
     assert "bad=0" in r.stdout
 
 
+def test_host_validation_sanitized(tmp_path):
+    """The tree and permutation checks that run before gs_upload_tree and
+    gs_apply_permutation (csrc/tree_check.h) on random BFS trees and their
+    mutations, against a brute-force statement of the same rules, built with
+    AddressSanitizer and UBSan."""
+    exe = tmp_path / "tree_check_fuzz"
+    subprocess.run(["g++", "-O1", "-g", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "native", "tree_check_fuzz.cpp")], check=True)
+    r = subprocess.run([str(exe), "200"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0" in r.stdout
+
+
+def test_oracle_sanitized(tmp_path):
+    """The oracle's C code built with AddressSanitizer and UBSan, run over the
+    edge inputs (N = 1/16/17, bucket 1, node-pool caps, 1e+-30 scales,
+    coincident and massless particles, NaN / Inf / ToInt32 wrap)."""
+    exe = tmp_path / "oracle_asan"
+    subprocess.run(["gcc", "-O1", "-g", "-std=c11", "-D_GNU_SOURCE", "-ffp-contract=off",
+                    "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+                    "-fno-omit-frame-pointer", "-o", str(exe),
+                    os.path.join(ROOT, "tests", "native", "oracle_asan.c"),
+                    os.path.join(ROOT, "oracle", "morton_ref.c"),
+                    os.path.join(ROOT, "oracle", "walk_ref.c"), "-lm", "-lpthread"], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr[-3000:]
+    assert "bad=0" in r.stdout
+
+
 def test_initial_conditions_deterministic():
     from galaxy_hip import ic
     a = ic.make("cpu_disk_16k")
