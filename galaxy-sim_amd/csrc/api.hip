@@ -502,6 +502,7 @@ void arena_place(gs_sim* s, HostSlot& h, uint64_t n) {
   h.d.noff = (uint32_t)nu;
   h.d.ncap = (uint32_t)nodes;
   h.d.max_nodes = nodes;
+  h.fixed_max_nodes = false;  // a new galaxy starts with the leaf rule of new MortonTree(n)
   h.d.nchunks = h.d.nwblocks = 0;
   s->max_ncap = std::max(s->max_ncap, h.d.ncap);
   s->layout_dirty = true;
