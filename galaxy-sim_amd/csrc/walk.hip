@@ -109,8 +109,9 @@ __device__ __forceinline__ void pair_fast(float4 q, float4 p, float eps2, float&
 // from the reference's, so only the default (tolerance) path uses it;
 // FASTLEAF semantics (the j == i pair adds r = 0) hold.
 struct WaveLds {
-  float4 pos[64];  // positions of the opening lanes, by rank
-  float4 sum[64];  // their sums (ax, ay, az, pot), by rank
+  float4 pos[64];   // positions of the opening lanes, by rank
+  float4 sum[64];   // their sums (ax, ay, az, pot), by rank
+  float4 junk[64];  // the stores of lanes that do not end a segment
 };
 
 template <int SH>
@@ -155,13 +156,28 @@ __device__ __forceinline__ void segment_sums(float& a, float& b, float& c, float
 #define GS_TR_ROUND 20u
 #define GS_TR_LCP 4u
 #define GS_TR_LEAF 12u
-__device__ __forceinline__ bool leaf_transposes(uint64_t open, uint32_t count, uint32_t& lcp) {
-  if (count < 2 || count > 32) return false;
-  lcp = 32u - (uint32_t)__builtin_clz(count - 1u);  // ceil(log2(count)), 1..5
-  const uint32_t per = 64u >> lcp;                  // opening lanes per round
-  const uint32_t k = (uint32_t)__builtin_popcountll(open);
-  const uint32_t rounds = (k + per - 1u) / per;
-  return rounds * (GS_TR_ROUND + GS_TR_LCP * lcp) + GS_TR_LEAF < 14u * count;
+// The rule as a table, one entry per lane (read with v_readlane, so the
+// choice costs the scalar unit one compare): lane c holds the most opening
+// lanes k for which a leaf of c members is summed transposed (0: never).  A
+// leaf of c <= 32 members padded to 2^lcp, opened by k lanes, takes
+// rounds(k) = ceil(k / per) rounds of 64 / 2^lcp = per opening lanes, and
+//   rounds(k) * (GS_TR_ROUND + GS_TR_LCP lcp) + GS_TR_LEAF < 14 c
+//   <=>  ceil(k / per) <= R  <=>  k <= R per,
+// R = floor((14 c - GS_TR_LEAF - 1) / (GS_TR_ROUND + GS_TR_LCP lcp)).
+__device__ __forceinline__ uint32_t tr_kmax(uint32_t c) {
+  if (c < 2u || c > 32u) return 0u;
+  const uint32_t lcp = 32u - (uint32_t)__builtin_clz(c - 1u);
+  const uint32_t per = 64u >> lcp;
+  const uint32_t r = (14u * c - GS_TR_LEAF - 1u) / (GS_TR_ROUND + GS_TR_LCP * lcp);
+  const uint32_t k = r * per;
+  return k > 64u ? 64u : k;
+}
+
+// the rule for a leaf of count members opened by the lanes `open`; trk is
+// tr_kmax of the lane's own index
+__device__ __forceinline__ bool leaf_transposes(uint64_t open, uint32_t count, uint32_t trk) {
+  const uint32_t kmax = (uint32_t)__builtin_amdgcn_readlane((int)trk, (int)(count <= 32u ? count : 0u));
+  return (uint32_t)__builtin_popcountll(open) <= kmax;
 }
 
 // the pairs of one leaf [first, first + count) for the opening lanes `open`
@@ -188,7 +204,11 @@ __device__ __forceinline__ void leaf_transposed(const float4* __restrict__ posm,
     float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
     pair_fast(q, pi, eps2, sx, sy, sz, sp);
     segment_sums<LCP>(sx, sy, sz, sp);
-    if (j == CP - 1u) L->sum[base + g] = make_float4(sx, sy, sz, sp);
+    // every lane stores, with no branch, so the sums stay live in every lane
+    // and each DPP stage is one fused add: a segment's last lane into the
+    // sums, the others into their own junk slot
+    float4* dst = j == CP - 1u ? &L->sum[base + g] : &L->junk[lane];
+    *dst = make_float4(sx, sy, sz, sp);
   }
   __builtin_amdgcn_wave_barrier();
   if (mine) {
@@ -222,6 +242,9 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
                                               float& ax, float& ay, float& az, float& pot,
                                               float& dens, uint32_t& cnt, WaveLds* wl) {
   int32_t alim = valid ? 0 : -1;  // active depth limit: active at depth d iff d <= alim
+  // this lane's entry of the transposed-leaf table
+  const uint32_t trk =
+      tr_kmax(__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)));
   const float selfpot = FASTLEAF ? p.w * __builtin_amdgcn_rsqf(eps2) : 0.f;
   int32_t nd = __builtin_amdgcn_ballot_w64(valid) ? 0 : -1;  // node link (record byte offset)
   // the wavefront's execution mask in the node loop (every lane of the
@@ -332,7 +355,8 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       uint32_t lcp = 0;
       if (need_mask == 0) {
         // no lane opens the leaf
-      } else if (FASTLEAF && leaf_transposes(need_mask, count, lcp)) {
+      } else if (FASTLEAF && leaf_transposes(need_mask, count, trk)) {
+        lcp = 32u - (uint32_t)__builtin_clz(count - 1u);  // ceil(log2(count)), 1..5
         const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 #define GS_TR(L) leaf_transposed<L>(posm, first, count, need_mask, lane, p, eps2, wl, ax, ay, az, pot)
         if (lcp == 1) GS_TR(1);
