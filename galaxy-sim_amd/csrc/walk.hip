@@ -112,6 +112,7 @@ struct WaveLds {
   float4 pos[64];   // positions of the opening lanes, by rank
   float4 sum[64];   // their sums (ax, ay, az, pot), by rank
   float4 junk[64];  // the stores of lanes that do not end a segment
+  float4 zero;      // read by the lanes that did not open the leaf
 };
 
 template <int SH>
@@ -191,7 +192,10 @@ __device__ __forceinline__ void leaf_transposed(const float4* __restrict__ posm,
   const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(open >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)open, 0u));
   const bool mine = __builtin_amdgcn_inverse_ballot_w64(open);
-  if (mine) L->pos[rank] = p;
+  // the opening lanes stage their positions by rank, the others write their
+  // junk slot (an address select instead of a branch)
+  float4* pdst = mine ? &L->pos[rank] : &L->junk[lane];
+  *pdst = p;
   const uint32_t j = lane & (CP - 1u), g = lane >> LCP;
   float4 q = posm[first + (j < count ? j : count - 1u)];
   if (j >= count) q.w = 0.f;  // padding member: no force, no potential
@@ -211,13 +215,13 @@ __device__ __forceinline__ void leaf_transposed(const float4* __restrict__ posm,
     *dst = make_float4(sx, sy, sz, sp);
   }
   __builtin_amdgcn_wave_barrier();
-  if (mine) {
-    const float4 t = L->sum[rank];
-    ax += t.x;
-    ay += t.y;
-    az += t.z;
-    pot += t.w;
-  }
+  // each opening lane adds its own sum; the others add the zero slot (x + 0
+  // leaves x, up to the sign of a zero)
+  const float4 t = *(mine ? &L->sum[rank] : &L->zero);
+  ax += t.x;
+  ay += t.y;
+  az += t.z;
+  pot += t.w;
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -436,6 +440,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   // block's wavefronts; only the default force path launches it)
   extern __shared__ WaveLds s_leaf[];
   WaveLds* wl = (EXACT || COUNT) ? nullptr : &s_leaf[w];
+  if (wl && lane == 0) wl->zero = make_float4(0.f, 0.f, 0.f, 0.f);
 
   // a workgroup never spans two slots (slot ranges are GS_SLOT_ALIGN-aligned).
   // The hardware deals workgroups to the 8 XCDs round-robin (XCD = id % 8);
