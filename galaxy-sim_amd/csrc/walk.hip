@@ -351,10 +351,13 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       // the leaf holding a lane's own particle sets its density.  FASTLEAF: a
       // lane that opens its own leaf takes back the self potential that the
       // member loop below adds.
-      if (__builtin_amdgcn_inverse_ballot_w64(act) && i - first < count) {
-        const float sz = nsize[gs_node(nd)];
-        dens = (float)count / ((sz * sz) * sz);
-        if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
+      const uint64_t own = act & __builtin_amdgcn_ballot_w64(i - first < count);
+      if (own != 0) {  // a scalar test first: most leaves hold no lane's particle
+        if (__builtin_amdgcn_inverse_ballot_w64(own)) {
+          const float sz = nsize[gs_node(nd)];
+          dens = (float)count / ((sz * sz) * sz);
+          if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
+        }
       }
       uint32_t lcp = 0;
       if (need_mask == 0) {
