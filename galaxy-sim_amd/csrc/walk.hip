@@ -181,26 +181,17 @@ __device__ __forceinline__ bool leaf_transposes(uint64_t open, uint32_t count, u
   return (uint32_t)__builtin_popcountll(open) <= kmax;
 }
 
-// the pairs of one leaf [first, first + count) for the opening lanes `open`
-// (every lane of the wavefront executes this; 2 <= count <= 2^LCP <= 32)
+// the rounds of one leaf [first, first + count) for the opening lanes
+// `open` (every lane of the wavefront executes this; 2 <= count <= 2^LCP <=
+// 32): each opening lane's sum ends in L->sum[its rank]
 template <int LCP>
-__device__ __forceinline__ void leaf_transposed(const float4* __restrict__ posm, uint32_t first,
-                                                uint32_t count, uint64_t open, uint32_t lane,
-                                                float4 p, float eps2, WaveLds* L, float& ax,
-                                                float& ay, float& az, float& pot) {
+__device__ __forceinline__ void leaf_rounds(const float4* __restrict__ posm, uint32_t first,
+                                            uint32_t count, uint32_t k, uint32_t lane,
+                                            float eps2, WaveLds* L) {
   constexpr uint32_t CP = 1u << LCP, PER = 64u >> LCP;
-  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(open >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)open, 0u));
-  const bool mine = __builtin_amdgcn_inverse_ballot_w64(open);
-  // the opening lanes stage their positions by rank, the others write their
-  // junk slot (an address select instead of a branch)
-  float4* pdst = mine ? &L->pos[rank] : &L->junk[lane];
-  *pdst = p;
   const uint32_t j = lane & (CP - 1u), g = lane >> LCP;
   float4 q = posm[first + (j < count ? j : count - 1u)];
   if (j >= count) q.w = 0.f;  // padding member: no force, no potential
-  const uint32_t k = (uint32_t)__builtin_popcountll(open);
-  __builtin_amdgcn_wave_barrier();
   // base + g <= 63: base is a multiple of PER below k <= 64.  Segments past
   // the last opening lane read stale positions; nobody reads their sums.
   for (uint32_t base = 0; base < k; base += PER) {
@@ -214,15 +205,6 @@ __device__ __forceinline__ void leaf_transposed(const float4* __restrict__ posm,
     float4* dst = j == CP - 1u ? &L->sum[base + g] : &L->junk[lane];
     *dst = make_float4(sx, sy, sz, sp);
   }
-  __builtin_amdgcn_wave_barrier();
-  // each opening lane adds its own sum; the others add the zero slot (x + 0
-  // leaves x, up to the sign of a zero)
-  const float4 t = *(mine ? &L->sum[rank] : &L->zero);
-  ax += t.x;
-  ay += t.y;
-  az += t.z;
-  pot += t.w;
-  __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -362,16 +344,40 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       uint32_t lcp = 0;
       if (need_mask == 0) {
         // no lane opens the leaf
-      } else if (FASTLEAF && leaf_transposes(need_mask, count, trk)) {
+      } else {
+      // FASTLEAF: the transposed form's sums come back in d, added below
+      // where the forms meet (d = 0 after the broadcast loop, which adds into
+      // the accumulators member by member): the accumulators keep one
+      // register on every path, so the forms merge without copies
+      float dx = 0.f, dy = 0.f, dz = 0.f, dp = 0.f;
+      if (FASTLEAF && leaf_transposes(need_mask, count, trk)) {
         lcp = 32u - (uint32_t)__builtin_clz(count - 1u);  // ceil(log2(count)), 1..5
         const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-#define GS_TR(L) leaf_transposed<L>(posm, first, count, need_mask, lane, p, eps2, wl, ax, ay, az, pot)
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need_mask >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)need_mask, 0u));
+        const bool mine = __builtin_amdgcn_inverse_ballot_w64(need_mask);
+        const uint32_t k = (uint32_t)__builtin_popcountll(need_mask);
+        // the opening lanes stage their positions by rank, the others write
+        // their junk slot (an address select instead of a branch)
+        float4* pdst = mine ? &wl->pos[rank] : &wl->junk[lane];
+        *pdst = p;
+        __builtin_amdgcn_wave_barrier();
+#define GS_TR(L) leaf_rounds<L>(posm, first, count, k, lane, eps2, wl)
         if (lcp == 1) GS_TR(1);
         else if (lcp == 2) GS_TR(2);
         else if (lcp == 3) GS_TR(3);
         else if (lcp == 4) GS_TR(4);
         else GS_TR(5);
 #undef GS_TR
+        __builtin_amdgcn_wave_barrier();
+        // each opening lane adds its own sum; the others add the zero slot (x +
+        // 0 leaves x, up to the sign of a zero)
+        const float4 t = *(mine ? &wl->sum[rank] : &wl->zero);
+        dx = t.x;
+        dy = t.y;
+        dz = t.z;
+        dp = t.w;
+        __builtin_amdgcn_wave_barrier();
       } else if (__builtin_amdgcn_inverse_ballot_w64(need_mask)) {  // lanes opening the leaf
         const uint32_t last = first + count;
         uint32_t j = first;
@@ -407,6 +413,13 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
           for (; j < last; j++)
             leaf_pair<EXACT, COUNT>(posm[j], j, i, p, G, eps2, ax, ay, az, pot, cnt);
         }
+      }
+      if (FASTLEAF) {
+        ax += dx;
+        ay += dy;
+        az += dz;
+        pot += dp;
+      }
       }
     }
     // next node: an opened internal node continues at its last child
