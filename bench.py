@@ -98,7 +98,8 @@ def cpu_reference_steps(data, c, steps, warmup=1):
         "step_s": float(lat.mean()), "p50_s": float(np.percentile(lat, 50)),
         "p99_s": float(np.percentile(lat, 99)),
         "stages_s": {k: v / steps for k, v in parts.items()},
-        "sample": (f"{steps} full steps of all {n} particles after {warmup} warm-up: "
+        "sample": (f"{steps} full steps of all {n} particles after {warmup} warm-up step(s) "
+                   f"from the initial particles: "
                    f"sortAndBuild + FP64 refresh on 1 thread, walk + kick-drift on {threads} "
                    f"threads"),
     }
@@ -153,14 +154,26 @@ def bench_config(name, steps, cpu_steps, fe_sample, args):
     try:
         sim.update_params(dt=c["dt"], G=1.0, epsilon=c["epsilon"], theta=c["theta"],
                           haloCenters=halo, **extra)
-        sim.step(args.warmup, sort_every=1)
-        sim.sync()
+        # every pass covers the same steps: the initial particles, the
+        # untimed warm-up steps, then the measured ones
+        def restart():
+            sim.upload(data)
+            sim.step(args.warmup, sort_every=1)
+            sim.sync()
+
+        restart()
         t0 = time.perf_counter()
         sim.step(steps, sort_every=1)  # back to back
         sim.sync()
         total = time.perf_counter() - t0
+        restart()
         lat = _timed_steps(sim, steps)  # each step synchronised: latency
-        inter = sim.count_interactions()
+        restart()
+        counts = []
+        for _ in range(steps):
+            sim.step(1, sort_every=1)
+            counts.append(sim.count_interactions())
+        inter = float(np.mean(counts))
     finally:
         sim.destroy()
     lat_ms = lat * 1e3
@@ -231,16 +244,27 @@ def bench_batch(args):
     pool = galaxy_hip.GalaxyPool(sizes, buckets=[bucket] * len(sizes), walk_wg=wg,
                                  radix_bits=radix)
     try:
-        for k, d in enumerate(datas):
-            pool.upload(k, d, **params)
-        pool.step(args.warmup, sort_every=1)
-        pool.sync()
+        # every pass, the plain forms' included, covers the same steps: the
+        # initial particles, the untimed warm-up steps, then the measured ones
+        def restart():
+            for k, d in enumerate(datas):
+                pool.upload(k, d, **params)
+            pool.step(args.warmup, sort_every=1)
+            pool.sync()
+
+        restart()
         t0 = time.perf_counter()
         pool.step(args.batch_steps, sort_every=1)  # back to back
         pool.sync()
         t_pool = time.perf_counter() - t0
+        restart()
         lat = _timed_steps(pool, args.batch_steps)  # each step synchronised: latency
-        inter = pool.count_interactions()
+        restart()
+        counts = []
+        for _ in range(args.batch_steps):
+            pool.step(1, sort_every=1)
+            counts.append(pool.count_interactions())
+        inter = float(np.mean(counts))
     finally:
         pool.destroy()
     pooled = total_n * args.batch_steps / t_pool
@@ -250,13 +274,13 @@ def bench_batch(args):
         return {"n_total": total_n, "value": pooled, "unit": "particle-steps/s",
                 "ms_per_step": t_pool / args.batch_steps * 1e3}
     # plain form 1: one galaxy after another, each on its own handle
-    ksteps = args.plain_steps
+    ksteps = args.batch_steps
     t_seq = 0.0
     for d in datas:
         sim = galaxy_hip.NBodyGPU(d, bucket=bucket, walk_wg=wg, radix_bits=radix)
         try:
             sim.update_params(**params)
-            sim.step(1, sort_every=1)
+            sim.step(args.warmup, sort_every=1)
             sim.sync()
             t0 = time.perf_counter()
             sim.step(ksteps, sort_every=1)
@@ -275,7 +299,7 @@ def bench_batch(args):
             sim.update_params(**params)
             sims.append(sim)
         for sim in sims:
-            sim.step(1, sort_every=1)
+            sim.step(args.warmup, sort_every=1)
         for sim in sims:
             sim.sync()
         t0 = time.perf_counter()
@@ -544,8 +568,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-plain", action="store_true",
                     help="batched run: time the pooled steps only (profiling)")
-    ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--plain-steps", type=int, default=5,
+                    help="steps timed per job for the served plain form (scaled to the job's steps)")
     ap.add_argument("--serve-steps", type=int, default=200)
     ap.add_argument("--serve-resident", type=int, default=8 << 20,
                     help="resident particle limit of the server")
@@ -640,13 +665,20 @@ def bench_headline(args):
     n, theta, eps, bucket = args.n, 0.5, 0.005, 32
     log(f"headline: collision_4m N={n}")
     data = ic.make("collision_4m", n=n)
+    # The collision evolves (the interactions per step grow by ~10 % over the
+    # first 40 steps), so every measurement below covers the same steps: the
+    # initial particles, W untimed steps, then the K measured steps.
     sim = galaxy_hip.NBodyGPU(data, bucket=bucket, timing=True, exact_force=args.exact)
     sim.update_params(dt=0.005, G=1.0, epsilon=eps, theta=theta)
-    for _ in range(args.warmup):
-        sim.step(1, sort_every=1)
-    sim.sync()
-    sim.stage_times(reset=True)
 
+    def restart(s):
+        s.upload(data)
+        for _ in range(args.warmup):
+            s.step(1, sort_every=1)
+        s.sync()
+        s.stage_times(reset=True)
+
+    restart(sim)
     # throughput: K steps launched back to back (the simulation loop), one
     # synchronisation at each end
     t_start = time.perf_counter()
@@ -654,22 +686,27 @@ def bench_headline(args):
     sim.sync()
     total = time.perf_counter() - t_start
     stages = sim.stage_times(reset=True)
-    # latency: K more steps, each followed by a synchronisation (p50/p99)
+    # latency: the same K steps, each followed by a synchronisation (p50/p99)
+    restart(sim)
     lat = _timed_steps(sim, args.steps)
-    sim.stage_times(reset=True)
+    # interactions of the same K steps, each counted right after its step
+    # (that step's tree against the particles one drift later)
+    restart(sim)
+    counts = []
+    for _ in range(args.steps):
+        sim.step(1, sort_every=1)
+        counts.append(sim.count_interactions())
+    inter = float(np.mean(counts))
     info = sim.tree_info()
-    inter = sim.count_interactions()
     sim.destroy()
     # stage breakdown: the timed steps replay two graphs (rebuild + refresh,
-    # walk) with HIP events between them; a few steps with direct launches
-    # time every stage
+    # walk) with HIP events between them; the same K steps with direct
+    # launches time every stage
     sim = galaxy_hip.NBodyGPU(data, bucket=bucket, timing=True, exact_force=args.exact,
                               graph=False)
     sim.update_params(dt=0.005, G=1.0, epsilon=eps, theta=theta)
-    sim.step(1, sort_every=1)
-    sim.sync()
-    sim.stage_times(reset=True)
-    for _ in range(5):
+    restart(sim)
+    for _ in range(args.steps):
         sim.step(1, sort_every=1)
         sim.sync()
     direct = sim.stage_times(reset=True)
@@ -707,6 +744,7 @@ def bench_headline(args):
             "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS, "traffic": None,
             "interactions_per_step": inter, "flop_per_interaction": WALK_FLOP_PER_INTERACTION,
+            "interactions_first_last": [int(counts[0]), int(counts[-1])],
             "walk_avg_ms": walk_avg_ms,
             "walk_algorithmic_bytes": (68 + 64 * nu) * n,
             # the walk against the HBM roofline too: its algorithmic bytes per
@@ -723,11 +761,15 @@ def bench_headline(args):
             "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "stages_ms": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in direct.items()},
-        "stages_note": "stages_ms: 5 steps with direct launches and an event pair per stage; "
-                       "the timed steps replay captured graphs",
-        "timing_note": "value and ms_per_step: K steps launched back to back with one "
-                       "synchronisation at each end; p50_ms / p99_ms: K more steps, each "
-                       "followed by a stream synchronisation",
+        "stages_note": "stages_ms: the K timed steps again with direct launches, an event "
+                       "pair per stage and a synchronisation per step; the timed steps replay "
+                       "captured graphs",
+        "timing_note": "every figure covers the same K steps from the initial particles after "
+                       "W untimed ones (the collision evolves); value and ms_per_step: the K "
+                       "steps launched back to back with one synchronisation at each end; "
+                       "p50_ms / p99_ms: the K steps again, each followed by a stream "
+                       "synchronisation; interactions_per_step: their mean, each step counted "
+                       "on a third pass",
         "interactions_per_s": inter / (ms_per_step * 1e-3),
         "cpu_baseline": None,
     }
@@ -740,7 +782,8 @@ def bench_headline(args):
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_steps} reference steps of all {n} particles on the host")
         c = dict(ic.CONFIGS["collision_4m"], n=n)
-        cb = cpu_reference_steps(data, c, args.cpu_steps)
+        # the first steps of the GPU's timed window
+        cb = cpu_reference_steps(data, c, args.cpu_steps, warmup=args.warmup)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["cpu_baseline"].update({k: cb[k] for k in ("step_s", "p50_s", "p99_s", "stages_s")})
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
