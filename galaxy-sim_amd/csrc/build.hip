@@ -41,6 +41,151 @@ __device__ __forceinline__ void st_agent64(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Nodes per tile: a level whose GS_BUILD_TILE-node tiles over all slots
+// number at most kFewTiles (a few blocks' work on a GPU of 256 CUs) takes
+// 32-node tiles (one row of 8 lanes x 32 nodes) instead, which spreads its
+// searches over 8 times as many blocks.  Small tiles need one look-back
+// word each, and a slot has ceil(ncap / GS_BUILD_TILE) of them.
+constexpr uint32_t kSmallTile = 32;
+constexpr uint32_t kFewTiles = 64;
+__device__ __forceinline__ uint32_t tile_nodes(uint32_t cnt, uint32_t ncap, bool small) {
+  const uint32_t words = (ncap + GS_BUILD_TILE - 1) / GS_BUILD_TILE;
+  return small && (cnt + kSmallTile - 1) / kSmallTile <= words ? kSmallTile
+                                                               : (uint32_t)GS_BUILD_TILE;
+}
+
+// Every loaded value passes through one empty asm: every load of the round
+// issues before the first wait.  Left alone, the compiler turns "select
+// between two loaded values" into "load at the selected address" (which
+// waits for one step before the next step's load) and interleaves the waits
+// of one search with the loads of the next.
+#define GS_V3(r) "+v"(v[r][0]), "+v"(v[r][1]), "+v"(v[r][2])
+__device__ __forceinline__ void loads_in_flight(uint32_t (&v)[8][3]) {
+  asm("" : GS_V3(0), GS_V3(1), GS_V3(2), GS_V3(3), GS_V3(4), GS_V3(5), GS_V3(6), GS_V3(7));
+}
+#undef GS_V3
+#define GS_V5(a) "+v"(v[0][a]), "+v"(v[0][a + 1]), "+v"(v[0][a + 2]), "+v"(v[0][a + 3]), \
+                 "+v"(v[0][a + 4])
+__device__ __forceinline__ void loads_in_flight(uint32_t (&v)[1][15]) {
+  asm("" : GS_V5(0), GS_V5(5), GS_V5(10));
+}
+#undef GS_V5
+
+// Phase 1 of a tile: octant boundaries b_0..b_7 (+ hi in s_b[j][8]) of its
+// nodes, 8 lanes per node (lane k of a node finds the first code with digit
+// >= k below the node's common prefix: a lower_bound over the node's
+// particles, whose digit is monotone).  Each thread owns one octant of R
+// nodes (rows of 32 nodes) and advances their R searches in lockstep, STEPS
+// bisection steps per round: the probes of all 2^STEPS - 1 decision-tree
+// nodes of every search are loaded together, so a round costs one memory
+// latency for STEPS steps of R searches.  Every probe lies inside the
+// node's particle range; a finished search (len 0) probes index 0, which
+// exists whenever a lane is still in the loop (its node is non-empty).
+template <int R, int STEPS>
+__device__ __forceinline__ void octant_bounds(const NodeRec* __restrict__ nodes,
+                                              const uint32_t* __restrict__ codes, uint32_t start,
+                                              uint32_t count, uint32_t first_local, uint32_t n,
+                                              uint32_t bucket, uint64_t max_nodes, int d,
+                                              uint32_t (*s_b)[9]) {
+  constexpr int P = (1 << STEPS) - 1;
+  const int shift = 27 - 3 * d;
+  const uint32_t k = threadIdx.x & 7;
+  uint32_t sfirst[R], slen[R], stgt[R];
+  bool more = false;
+  // the R node records, then the first code of each node, are loaded
+  // together (rows past the level's end load the level's first node, and
+  // nodes without a valid range load code 0 of a non-empty slot), so the
+  // set-up costs two memory latencies rather than 2R
+  int4 infr[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t local = first_local + (uint32_t)r * 32 + (threadIdx.x >> 3);
+    infr[r] = nodes[start + (local < count ? local : 0u)].info;
+  }
+  uint32_t c0[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t lo = (uint32_t)infr[r].y;
+    c0[r] = 0;
+    if (n > 0) c0[r] = codes[lo < n ? lo : 0u];  // n is uniform over the block
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
+    const uint32_t local = first_local + j;
+    sfirst[r] = 0;
+    slen[r] = 0;
+    stgt[r] = 0;
+    if (local < count) {
+      const uint32_t idx = start + local;
+      const int4 inf = infr[r];
+      const uint32_t lo = (uint32_t)inf.y, cnt = gs_meta_count((uint32_t)inf.z);
+      const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
+      sfirst[r] = lo;
+      if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
+        slen[r] = cnt;
+        stgt[r] = ((c0[r] >> (shift + 3)) << (shift + 3)) | (k << shift);
+        more = true;
+      }
+      if (k == 0) s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
+    }
+  }
+  while (more) {
+    uint32_t v[R][P];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      // (first, len) of every decision-tree node in heap order
+      uint32_t pf[P], pl[P];
+      pf[0] = sfirst[r];
+      pl[0] = slen[r];
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        const uint32_t l = pl[q], h = l >> 1;
+        v[r][q] = codes[l > 0 ? pf[q] + h : 0u];
+        if (2 * q + 2 < P) {
+          pf[2 * q + 1] = pf[q];
+          pl[2 * q + 1] = h;
+          pf[2 * q + 2] = pf[q] + h + 1;
+          pl[2 * q + 2] = l > 0 ? l - h - 1 : 0u;
+        }
+      }
+    }
+    loads_in_flight(v);
+    // the STEPS steps as selects (a finished search keeps len 0 and its first)
+    more = false;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      // v[r] holds the remaining decision tree in heap order; each step
+      // moves the taken child's subtree to the front (level lv's half to
+      // level lv - 1, in place), so every index is a constant
+      uint32_t f = sfirst[r], l = slen[r];
+#pragma unroll
+      for (int st = 0; st < STEPS; st++) {
+        const uint32_t h = l >> 1;
+        const bool right = l > 0 && v[r][0] < stgt[r];
+        f = right ? f + h + 1 : f;
+        l = right ? l - h - 1 : h;
+#pragma unroll
+        for (int lv = 1; lv < STEPS - st; lv++) {
+#pragma unroll
+          for (int i = 0; i < (1 << (lv - 1)); i++) {
+            const int src = (1 << lv) - 1 + i;
+            v[r][(1 << (lv - 1)) - 1 + i] = right ? v[r][src + (1 << (lv - 1))] : v[r][src];
+          }
+        }
+      }
+      sfirst[r] = f;
+      slen[r] = l;
+      more |= l > 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
+    if (first_local + j < count) s_b[j][k] = sfirst[r];
+  }
+}
+
 __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const uint32_t* __restrict__ keys, NodeRec* __restrict__ pool, float* __restrict__ nsize_pool,
     const DevSlot* __restrict__ slots, DevScalars* scs, DevBatch* batch, uint32_t nslots, int d,
@@ -54,17 +199,25 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t k = gtid; k < clear_n; k += gridDim.x * blockDim.x) status_clear[k] = 0;
 
-  // tiles per slot at this level; slots without nodes here end their tree
-  uint32_t my_tiles = 0;
+  // nodes, then tiles per slot at this level; slots without nodes here end
+  // their tree
+  uint32_t cnt = 0, ncap = 0;
   if (threadIdx.x < nslots) {
-    const uint32_t cnt = scs[threadIdx.x].level_count[d];
-    my_tiles = (cnt + GS_BUILD_TILE - 1) / GS_BUILD_TILE;
+    cnt = scs[threadIdx.x].level_count[d];
+    ncap = slots[threadIdx.x].ncap;
     if (cnt == 0 && blockIdx.x == 0) {
       scs[threadIdx.x].level_start[d + 1] = scs[threadIdx.x].level_start[d];
       scs[threadIdx.x].level_count[d + 1] = 0;
     }
   }
-  gs_slot_prefix(my_tiles, nslots, s_pre);
+  gs_slot_prefix((cnt + GS_BUILD_TILE - 1) / GS_BUILD_TILE, nslots, s_pre);
+  // the level's tile total, the same in every block (read before the next
+  // prefix, which rewrites s_pre only after a barrier every thread reaches)
+  const bool small = s_pre[nslots] <= kFewTiles;
+  if (small) {
+    const uint32_t my_tiles = cnt == 0 ? 0u : (cnt - 1) / tile_nodes(cnt, ncap, true) + 1;
+    gs_slot_prefix(my_tiles, nslots, s_pre);
+  }
   const uint32_t total_tiles = s_pre[nslots];
   // the grid is sized for the largest level: blocks beyond this level's tile
   // count leave without touching the ticket counter (tickets keep the
@@ -74,7 +227,6 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
   // total_tiles and their first tickets are 0 .. total_tiles - 1: each block
   // builds one tile and leaves without asking for another
   const bool one_tile = total_tiles <= gridDim.x;
-  const int shift = 27 - 3 * d;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 
   for (;;) {
@@ -102,110 +254,17 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const float size_c = sc->level_size[d + 1];
     const int s2_c = (int)sc->level_thr[d + 1];
 
-    // phase 1: octant boundaries b_0..b_7 (+ hi), 8 lanes per node; each
-    // thread owns one octant of R nodes and advances their R binary searches
-    // in lockstep, so the dependent loads of the searches overlap
-    {
-      constexpr int R = GS_BUILD_TILE / 32;
-      const uint32_t k = threadIdx.x & 7;
-      uint32_t sfirst[R], slen[R], stgt[R];
-      bool more = false;
-      // the R node records, then the first code of each node, are loaded
-      // together (rows past the level's end load the level's first node, and
-      // nodes without a valid range load code 0 of a non-empty slot), so the
-      // set-up costs two memory latencies rather than 2R
-      int4 infr[R];
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const uint32_t local = tile * GS_BUILD_TILE + (uint32_t)r * 32 + (threadIdx.x >> 3);
-        infr[r] = nodes[start + (local < count ? local : 0u)].info;
-      }
-      uint32_t c0[R];
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const uint32_t lo = (uint32_t)infr[r].y;
-        c0[r] = 0;
-        if (n > 0) c0[r] = codes[lo < n ? lo : 0u];  // n is uniform over the block
-      }
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
-        const uint32_t local = tile * GS_BUILD_TILE + j;
-        sfirst[r] = 0;
-        slen[r] = 0;
-        stgt[r] = 0;
-        if (local < count) {
-          const uint32_t idx = start + local;
-          const int4 inf = infr[r];
-          const uint32_t lo = (uint32_t)inf.y, cnt = gs_meta_count((uint32_t)inf.z);
-          const bool leaf = cnt <= bucket || d >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
-          sfirst[r] = lo;
-          if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
-            slen[r] = cnt;
-            stgt[r] = ((c0[r] >> (shift + 3)) << (shift + 3)) | (k << shift);
-            more = true;
-          }
-          if (k == 0) s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
-        }
-      }
-      // lower_bound over [first, first + len) for every r at once, two
-      // bisection steps per round: the probe of the first step and both
-      // possible probes of the second are loaded together, with no branch
-      // between the loads, so a round costs one memory latency for all R
-      // searches and both steps.  Every probe lies in [first, first + len),
-      // inside the node's particle range; a finished search (len 0) probes
-      // index 0, which exists whenever a lane is still in the loop (its node
-      // is non-empty).
-      while (more) {
-        uint32_t vm[R], vl[R], vr[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const uint32_t len = slen[r], half = len >> 1, rest = len - half - 1;
-          const uint32_t am = len > 0 ? sfirst[r] + half : 0u;
-          const uint32_t al = half > 0 ? sfirst[r] + (half >> 1) : am;
-          const uint32_t ar = len > 0 && rest > 0 ? am + 1 + (rest >> 1) : am;
-          vm[r] = codes[am];
-          vl[r] = codes[al];
-          vr[r] = codes[ar];
-        }
-        // all 3R loaded values pass through one empty asm: every load issues
-        // before the first wait.  Left alone, the compiler turns "select
-        // between two loaded values" into "load at the selected address"
-        // (which waits for the first step before the second step's load) and
-        // interleaves the waits of one search with the loads of the next.
-        static_assert(R == 8, "the asm below names 8 searches");
-#define GS_P3(r) "+v"(vm[r]), "+v"(vl[r]), "+v"(vr[r])
-        asm("" : GS_P3(0), GS_P3(1), GS_P3(2), GS_P3(3), GS_P3(4), GS_P3(5), GS_P3(6), GS_P3(7));
-#undef GS_P3
-        // both steps as selects (a finished search keeps len 0 and its first)
-        more = false;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const uint32_t len = slen[r], half = len >> 1;
-          const bool right = len > 0 && vm[r] < stgt[r];
-          uint32_t f = right ? sfirst[r] + half + 1 : sfirst[r];
-          uint32_t l = right ? len - half - 1 : half;
-          const uint32_t v2 = right ? vr[r] : vl[r];
-          const uint32_t h2 = l >> 1;
-          const bool right2 = l > 0 && v2 < stgt[r];
-          f = right2 ? f + h2 + 1 : f;
-          l = right2 ? l - h2 - 1 : h2;
-          sfirst[r] = f;
-          slen[r] = l;
-          more |= l > 0;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const uint32_t j = (uint32_t)r * 32 + (threadIdx.x >> 3);
-        if (tile * GS_BUILD_TILE + j < count) s_b[j][k] = sfirst[r];
-      }
-    }
+    const uint32_t tn = tile_nodes(count, cap, small);
+    if (tn == GS_BUILD_TILE)
+      octant_bounds<GS_BUILD_TILE / 32, 2>(nodes, codes, start, count, tile * tn, n, bucket,
+                                            max_nodes, d, s_b);
+    else
+      octant_bounds<1, 4>(nodes, codes, start, count, tile * tn, n, bucket, max_nodes, d, s_b);
     __syncthreads();
 
     // phase 2: one node per thread
-    const uint32_t local = tile * GS_BUILD_TILE + threadIdx.x;
-    const bool valid = local < count;
+    const uint32_t local = tile * tn + threadIdx.x;
+    const bool valid = threadIdx.x < tn && local < count;
     const uint32_t idx = start + local;
     uint32_t mask = 0, nchild = 0;
     bool leaf = true;
