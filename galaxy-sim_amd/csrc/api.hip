@@ -242,7 +242,7 @@ void launch_sort(gs_sim* s, bool keep_raw) {
   if (s->stel[0]) s->stel_cur ^= 1;  // the gather moved the stellar records too
   {
     Stage t(s, GS_T_BUILD);
-    check(gs_launch_build(s), "build");
+    check(gs_launch_build(s, true), "build");
   }
 }
 
@@ -1437,7 +1437,8 @@ int gs_tree_build(gs_sim* s, const uint32_t* sorted_codes, const float* pos3, co
           "root");
     check(hipMemcpyAsync(&s->nsize[h.d.noff], &f0, sizeof f0, hipMemcpyHostToDevice, s->stream),
           "root size");
-    check(gs_launch_build(s), "build");
+    // the codes come from the caller: no histogram of them
+    check(gs_launch_build(s, false), "build");
     check(gs_launch_refresh(s, s->posm[s->cur]), "refresh");
     h.have_tree = true;
     h.have_com = true;

@@ -86,16 +86,17 @@ __device__ __forceinline__ void octant_bounds(const NodeRec* __restrict__ nodes,
                                               const uint32_t* __restrict__ codes, uint32_t start,
                                               uint32_t count, uint32_t first_local, uint32_t n,
                                               uint32_t bucket, uint64_t max_nodes, int d,
-                                              uint32_t (*s_b)[9]) {
+                                              uint32_t (*s_b)[9], int4* s_info) {
   constexpr int P = (1 << STEPS) - 1;
   const int shift = 27 - 3 * d;
   const uint32_t k = threadIdx.x & 7;
-  uint32_t sfirst[R], slen[R], stgt[R];
+  uint32_t sfirst[R], slen[R];
   bool more = false;
-  // the R node records, then the first code of each node, are loaded
-  // together (rows past the level's end load the level's first node, and
-  // nodes without a valid range load code 0 of a non-empty slot), so the
-  // set-up costs two memory latencies rather than 2R
+  // the R node records are loaded together (rows past the level's end load
+  // the level's first node); the first code of each node (nodes without a
+  // valid range load code 0 of a non-empty slot) is consumed only after the
+  // first round's probes have issued, so set-up and first round share a
+  // memory latency
   int4 infr[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
@@ -115,7 +116,6 @@ __device__ __forceinline__ void octant_bounds(const NodeRec* __restrict__ nodes,
     const uint32_t local = first_local + j;
     sfirst[r] = 0;
     slen[r] = 0;
-    stgt[r] = 0;
     if (local < count) {
       const uint32_t idx = start + local;
       const int4 inf = infr[r];
@@ -124,10 +124,12 @@ __device__ __forceinline__ void octant_bounds(const NodeRec* __restrict__ nodes,
       sfirst[r] = lo;
       if (!leaf && k > 0 && lo < n && cnt <= n - lo) {
         slen[r] = cnt;
-        stgt[r] = ((c0[r] >> (shift + 3)) << (shift + 3)) | (k << shift);
         more = true;
       }
-      if (k == 0) s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
+      if (k == 0) {
+        s_b[j][8] = leaf ? 0xffffffffu : lo + cnt;
+        s_info[j] = inf;  // phase 2 reads the record from here
+      }
     }
   }
   while (more) {
@@ -158,11 +160,13 @@ __device__ __forceinline__ void octant_bounds(const NodeRec* __restrict__ nodes,
       // v[r] holds the remaining decision tree in heap order; each step
       // moves the taken child's subtree to the front (level lv's half to
       // level lv - 1, in place), so every index is a constant
+      // the target: the node's common prefix above this level's digit, then k
+      const uint32_t tgt = ((c0[r] >> (shift + 3)) << (shift + 3)) | (k << shift);
       uint32_t f = sfirst[r], l = slen[r];
 #pragma unroll
       for (int st = 0; st < STEPS; st++) {
         const uint32_t h = l >> 1;
-        const bool right = l > 0 && v[r][0] < stgt[r];
+        const bool right = l > 0 && v[r][0] < tgt;
         f = right ? f + h + 1 : f;
         l = right ? l - h - 1 : h;
 #pragma unroll
@@ -192,6 +196,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     uint64_t* status, uint64_t* status_clear, uint32_t clear_n) {
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_b[GS_BUILD_TILE][9];
+  __shared__ int4 s_info[GS_BUILD_TILE];
   __shared__ uint32_t s_wsum[GS_BUILD_TILE / 64];
   __shared__ uint64_t s_prefix;
   __shared__ uint32_t s_pre[GS_MAX_SLOTS + 1 + GS_MAX_SLOTS / 64];
@@ -257,9 +262,10 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     const uint32_t tn = tile_nodes(count, cap, small);
     if (tn == GS_BUILD_TILE)
       octant_bounds<GS_BUILD_TILE / 32, 2>(nodes, codes, start, count, tile * tn, n, bucket,
-                                            max_nodes, d, s_b);
+                                            max_nodes, d, s_b, s_info);
     else
-      octant_bounds<1, 4>(nodes, codes, start, count, tile * tn, n, bucket, max_nodes, d, s_b);
+      octant_bounds<1, 4>(nodes, codes, start, count, tile * tn, n, bucket, max_nodes, d, s_b,
+                          s_info);
     __syncthreads();
 
     // phase 2: one node per thread
@@ -350,7 +356,7 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
     __syncthreads();
 
     if (valid) {
-      const int4 inf = nodes[idx].info;
+      const int4 inf = s_info[threadIdx.x];
       const int32_t skip = inf.w;
       if (leaf) {
         nodes[idx].info = make_int4(s2_d, inf.y,
@@ -384,12 +390,198 @@ __global__ __launch_bounds__(GS_BUILD_TILE) void build_level_kernel(
   }
 }
 
+// ---- levels 0 and 1 from the encode kernel's digit histogram -------------
+// The encode kernel counts the digits of every key for every radix pass.  The
+// top pass's digits hold key bits 24..29 (8-bit digits: bits 24..31, whose
+// two highest bits are 0 for every 30-bit code; the padding keys all fall in
+// digit 255; 10-bit digits: bits 20..29, padding in digit 1023), so the
+// number of keys per value of their two highest 3-bit digits -- 64 cells --
+// follows from the histogram, and a prefix over the cells gives the sorted
+// position of every cell's first key.  The root's octant boundaries are the
+// prefixes at cells 8k and those of the level-1 node of octant k the
+// prefixes at cells 8k + m: exactly the lower_bound results of
+// build_level_kernel for d = 0 and 1 (mortonTree.js:130-150), found without
+// reading a key.  One wavefront per slot writes the records, level ranges and
+// counts that those two launches write; blocks past the slots only clear.
+template <int BITS>
+__global__ __launch_bounds__(64) void build_top_kernel(NodeRec* __restrict__ pool,
+                                                       float* __restrict__ nsize_pool,
+                                                       const DevSlot* __restrict__ slots,
+                                                       DevScalars* scs, DevBatch* batch,
+                                                       uint32_t nslots, uint64_t* status_clear,
+                                                       uint32_t clear_n) {
+  __shared__ uint32_t s_ex[65];
+  // the level-2 launch's look-back words
+  for (uint32_t k = blockIdx.x * 64u + threadIdx.x; k < clear_n; k += gridDim.x * 64u)
+    status_clear[k] = 0;
+  if (blockIdx.x >= nslots) return;
+  DevScalars* sc = scs + blockIdx.x;
+  // an empty slot: the root kernel left every level empty, as the level
+  // launches would
+  if (sc->level_count[0] == 0) return;
+  const DevSlot sl = slots[blockIdx.x];
+  NodeRec* nodes = pool + sl.noff;
+  float* nsize = nsize_pool + sl.noff;
+  const uint32_t n = sl.n, bucket = sl.bucket, cap = sl.ncap;
+  const uint64_t max_nodes = sl.max_nodes;
+  const uint32_t lane = threadIdx.x;
+
+  // keys per cell; cell 63 is what the other cells leave of n (with 10-bit
+  // digits the padding keys share its last digit)
+  constexpr int top = gs_sort_passes(BITS) - 1;
+  uint32_t c = 0;
+  if (lane < 63) {
+    if (BITS == 8) {
+      c = sc->sort_hist[top * 256 + lane];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) c += sc->sort_hist[top * 1024 + lane * 16 + i];
+    }
+  }
+  uint32_t x = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if ((int)lane >= o) x += y;
+  }
+  s_ex[lane] = x - c;  // first sorted position of the cell's keys
+  if (lane == 0) s_ex[64] = n;
+  __syncthreads();
+
+  const int thr0 = (int)sc->level_thr[0], thr1 = (int)sc->level_thr[1];
+  const int thr2 = (int)sc->level_thr[2];
+  const float size1 = sc->level_size[1], size2 = sc->level_size[2];
+  uint32_t st = 0;  // status flags
+
+  // level 0: the root record of the root kernel covers [0, n) with skip -1
+  const int32_t rskip = -1;
+  const bool rleaf = n <= bucket || 0 >= GS_MAX_LEVEL || 9ull > max_nodes;
+  uint32_t mask0 = 0;
+  if (!rleaf) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (s_ex[8 * k + 8] > s_ex[8 * k]) mask0 |= 1u << k;
+  }
+  const uint32_t nch0 = __popc(mask0), fc0 = 1;  // level 1 starts after the root
+  uint32_t cnt1 = nch0;
+  if (1ull + nch0 > cap) {
+    st |= GS_ST_NODE_POOL;
+    cnt1 = cap > 1 ? cap - 1 : 0;
+  }
+  if (lane < 8 && ((mask0 >> lane) & 1u)) {
+    const uint32_t c1 = fc0 + __popc(mask0 & ((1u << lane) - 1u));
+    if (c1 < cap) {
+      // pre-order with octant 7 first: after child c comes c - 1
+      nsize[c1] = size1;
+      nodes[c1].info = make_int4(thr1, (int)s_ex[8 * lane],
+                                 (int)gs_meta_open(1, s_ex[8 * lane + 8] - s_ex[8 * lane]),
+                                 c1 > fc0 ? gs_link(c1 - 1) : rskip);
+    }
+  }
+  if (lane == 0)
+    nodes[0].info = rleaf ? make_int4(thr0, 0, (int)gs_meta_leaf(0, n), rskip)
+                          : make_int4(thr0, gs_link(fc0 + nch0 - 1),
+                                      (int)gs_meta_internal(0, mask0, nch0), rskip);
+
+  // level 1: the node of root octant k is the rank of k among the root's
+  // children (those that fitted the pool)
+  const uint32_t next1 = fc0 + cnt1;  // level 2 starts here
+  uint32_t mask1[8], excl1[8], total1 = 0;
+  bool leaf1[8], valid1[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t rank = __popc(mask0 & ((1u << k) - 1u));
+    valid1[k] = ((mask0 >> k) & 1u) && rank < cnt1;
+    const uint32_t idx = fc0 + rank, cnt = s_ex[8 * k + 8] - s_ex[8 * k];
+    leaf1[k] = cnt <= bucket || 1 >= GS_MAX_LEVEL || (uint64_t)idx + 9 > max_nodes;
+    mask1[k] = 0;
+    if (valid1[k] && !leaf1[k]) {
+#pragma unroll
+      for (int m = 0; m < 8; m++)
+        if (s_ex[8 * k + m + 1] > s_ex[8 * k + m]) mask1[k] |= 1u << m;
+    }
+    excl1[k] = total1;
+    total1 += __popc(mask1[k]);
+  }
+  {
+    // this lane's level-1 node (lanes 0..7) and level-2 child (cell = lane)
+    const uint32_t k = lane >> 3, m = lane & 7u;
+    uint32_t mk = 0, ek = 0;
+    bool vk = false, lk = true;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if ((uint32_t)q == k) {
+        mk = mask1[q];
+        ek = excl1[q];
+        vk = valid1[q];
+        lk = leaf1[q];
+      }
+    const uint32_t idx = fc0 + __popc(mask0 & ((1u << k) - 1u));
+    const int32_t skip_k = idx > fc0 ? gs_link(idx - 1) : rskip;
+    const uint32_t fc = next1 + ek, nch = __popc(mk);
+    if (vk && ((mk >> m) & 1u)) {
+      const uint32_t c2 = fc + __popc(mk & ((1u << m) - 1u));
+      if (c2 < cap) {
+        nsize[c2] = size2;
+        nodes[c2].info = make_int4(thr2, (int)s_ex[lane],
+                                   (int)gs_meta_open(2, s_ex[lane + 1] - s_ex[lane]),
+                                   c2 > fc ? gs_link(c2 - 1) : skip_k);
+      }
+    }
+    if (m == 0 && vk) {
+      const uint32_t lo = s_ex[8 * k], cnt = s_ex[8 * k + 8] - lo;
+      nodes[idx].info = lk ? make_int4(thr1, (int)lo, (int)gs_meta_leaf(1, cnt), skip_k)
+                           : make_int4(thr1, gs_link(fc + nch - 1),
+                                       (int)gs_meta_internal(1, mk, nch), skip_k);
+    }
+  }
+  if (lane == 0) {
+    if (cnt1 == 0) {  // level 1 is empty: the tree ends at the root
+      sc->level_start[1] = fc0;
+      sc->level_count[1] = 0;
+      sc->level_start[2] = fc0;
+      sc->level_count[2] = 0;
+      sc->num_nodes = fc0;
+      sc->num_levels = 1;
+    } else {
+      uint32_t cnt2 = total1;
+      if ((uint64_t)next1 + total1 > cap) {
+        st |= GS_ST_NODE_POOL;
+        cnt2 = cap > next1 ? cap - next1 : 0;
+      }
+      sc->level_start[1] = fc0;
+      sc->level_count[1] = cnt1;
+      sc->level_start[2] = next1;
+      sc->level_count[2] = cnt2;
+      sc->num_nodes = next1 + cnt2;
+      sc->num_levels = 2;
+    }
+    if (st) atomicOr(&batch->status, st);
+  }
+}
+
 }  // namespace
 
-hipError_t gs_launch_build(gs_sim* s) {
+hipError_t gs_launch_build(gs_sim* s, bool top_from_hist) {
   uint32_t grid = s->build_words < 1024 ? s->build_words : 1024;
   if (grid < 1) grid = 1;
-  for (int d = 0; d <= GS_MAX_LEVEL; d++) {
+  int d0 = 0;
+  if (top_from_hist) {
+    // levels 0 and 1 in one launch (the level-2 launch uses build_status[0])
+    const uint32_t tg = s->nslots > 32 ? s->nslots : 32;
+    if (s->radix_bits == 10)
+      build_top_kernel<10><<<tg, 64, 0, s->stream>>>(s->nodes, s->nsize, s->slots, s->sc,
+                                                     s->batch, s->nslots, s->build_status[0],
+                                                     s->build_words);
+    else
+      build_top_kernel<8><<<tg, 64, 0, s->stream>>>(s->nodes, s->nsize, s->slots, s->sc,
+                                                    s->batch, s->nslots, s->build_status[0],
+                                                    s->build_words);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    d0 = 2;
+  }
+  for (int d = d0; d <= GS_MAX_LEVEL; d++) {
     build_level_kernel<<<grid, GS_BUILD_TILE, 0, s->stream>>>(
         s->keys[gs_sorted_buf(s)], s->nodes, s->nsize, s->slots, s->sc, s->batch, s->nslots, d,
         s->build_status[d & 1],

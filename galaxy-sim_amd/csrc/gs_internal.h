@@ -401,7 +401,9 @@ hipError_t gs_launch_hist(gs_sim* s);  // histograms of host-supplied keys (slot
 hipError_t gs_launch_hist_bufs(const SortBufs& b, const uint32_t* src, uint32_t cap);
 hipError_t gs_launch_sort(gs_sim* s, bool gather);
 hipError_t gs_launch_sort_bufs(const SortBufs& b);  // every pass, no gather
-hipError_t gs_launch_build(gs_sim* s);
+// top_from_hist: the digit histograms belong to the sorted keys (an encode
+// counted them), so levels 0 and 1 come from them
+hipError_t gs_launch_build(gs_sim* s, bool top_from_hist);
 hipError_t gs_launch_refresh(gs_sim* s, const float4* posm);
 hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count);
 hipError_t gs_launch_reduce_final(gs_sim* s, uint32_t slot, double* out_dev);

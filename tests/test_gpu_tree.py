@@ -236,3 +236,33 @@ def test_locate_follows_particles_through_the_sort():
             sim.locate(30000)
     finally:
         sim.destroy()
+
+
+@pytest.mark.parametrize("radix", [8, 10])
+def test_top_levels_from_the_digit_histogram(radix):
+    """The step's build takes levels 0 and 1 from the encode kernel's digit
+    histogram instead of searching the keys; the leaf rules still apply node
+    by node there: maxNodes values where the pool rule (idx + 9 > maxNodes)
+    makes the root, all level-1 nodes or only the later ones leaves, and N
+    around the bucket (root leaf, one level).  Bit-exact against the oracle,
+    for 8- and 10-bit digits (10-bit: the padding keys share the top cell's
+    last digit)."""
+    base = ic.make("cpu_disk_16k", n=4096)
+    runs = [(base, mx) for mx in (5, 9, 10, 11, 13, 26, None)]
+    runs += [(base[:n].copy(), None) for n in (1, 2, 16, 17, 40, 300)]
+    for data, mx in runs:
+        n = data.shape[0]
+        banner(f"top levels radix {radix} N={n} maxNodes={mx}")
+        ref = oracle_tree(data, bucket=16, max_nodes=mx)
+        kw = {} if mx is None else {"max_nodes": mx}
+        sim = galaxy_hip.NBodyGPU(data, bucket=16, radix_bits=radix, **kw)
+        try:
+            sim.sort()
+            sim.refresh()
+            info = sim.tree_info()
+            nodes = sim.nodes()
+        finally:
+            sim.destroy()
+        assert info["num_nodes"] == ref["num_nodes"]
+        assert info["level_ranges"] == ref["level_ranges"]
+        assert np.array_equal(nodes, ref["nodes"])
