@@ -379,12 +379,14 @@ void push_tables(gs_sim* s, const std::vector<uint2>& chunk_tab,
 // arrays and tables at their allocated sizes, the per-slot records, stellar
 // records when present and the node staging.
 uint64_t pool_bytes(uint64_t P, uint64_t C, uint64_t chunks, uint64_t build_words,
-                    uint64_t walk_blocks, uint32_t nslots, bool stellar, uint64_t node64) {
+                    uint64_t walk_blocks, uint64_t walk_waves, uint32_t nslots, bool stellar,
+                    uint64_t node64) {
   return P * GS_ARENA_PARTICLE_BYTES + C * GS_ARENA_NODE_BYTES +
          2 * build_words * 8 +                   // build look-back words (ping-pong)
          chunks * (GS_HIST_WORDS * 4 + 8 * 4 +   // sort look-back words, bbox partials
                    sizeof(uint2) + 4) +          // chunk and tile tables
-         walk_blocks * (4 + GS_NRED * 8) +       // walk block table, reduction partials
+         walk_blocks * 4 +                       // walk block table
+         walk_waves * GS_NRED * 8 +              // reduction partials (one set per wavefront)
          (P / GS_BIG_LEAF + 64) * sizeof(uint2) +  // large-leaf list
          (uint64_t)nslots * (sizeof(DevSlot) + sizeof(DevScalars) + sizeof(DevParams) + 4) +
          sizeof(DevBatch) + GS_NRED * 8 + (stellar ? P * 32 : 0) + node64 * 64;
@@ -616,7 +618,8 @@ gs_sim* make_pool(const uint32_t* caps, const uint32_t* buckets, uint32_t nslots
     dalloc(&s->acc, P, "acc");
     dalloc(&s->dens, P, "density");
     dalloc(&s->inter, P, "interactions");
-    dalloc(&s->red_part, (size_t)s->cap_walk_blocks * GS_NRED, "reductions");
+    // one set of FP64 partials per walk wavefront
+    dalloc(&s->red_part, (size_t)s->cap_walk_blocks * (s->walk_wg / 64) * GS_NRED, "reductions");
     dalloc(&s->red_out, GS_NRED, "reduction");
     dalloc(&s->slots, nslots, "slots");
     dalloc(&s->sc, nslots, "scalars");
@@ -822,7 +825,8 @@ int gs_pool_info(gs_sim* s, uint32_t* nslots, uint64_t* device_bytes) {
     if (nslots) *nslots = s->nslots;
     if (device_bytes)
       *device_bytes = pool_bytes(s->ptot, s->ntot, s->cap_chunks, s->cap_build_words,
-                                 s->cap_walk_blocks, s->nslots, s->stel[0] != nullptr,
+                                 s->cap_walk_blocks, (uint64_t)s->cap_walk_blocks * (s->walk_wg / 64),
+                                 s->nslots, s->stel[0] != nullptr,
                                  s->node64 ? s->node64_cap : 0);
   });
 }
@@ -840,7 +844,7 @@ int gs_create_arena(const gs_arena_config* cfg, gs_sim** out) {
   const uint64_t ns = cfg->nslots;
   auto bytes = [&](uint64_t P) {
     const uint64_t C = 2 * P + 4096 * ns;
-    return pool_bytes(P, C, P / GS_CHUNK + ns, C / GS_BUILD_TILE + ns, P / wg, cfg->nslots, false,
+    return pool_bytes(P, C, P / GS_CHUNK + ns, C / GS_BUILD_TILE + ns, P / wg, P / 64, cfg->nslots, false,
                       0);
   };
   const uint64_t per = GS_ARENA_PARTICLE_BYTES + 2 * GS_ARENA_NODE_BYTES;
@@ -868,7 +872,8 @@ int gs_arena_info(gs_sim* s, uint64_t* device_bytes, uint64_t* used_bytes, uint6
     need(s->arena, GS_E_INVALID, "not an arena pool");
     if (device_bytes)
       *device_bytes = pool_bytes(s->ptot, s->ntot, s->cap_chunks, s->cap_build_words,
-                                 s->cap_walk_blocks, s->nslots, false,
+                                 s->cap_walk_blocks, (uint64_t)s->cap_walk_blocks * (s->walk_wg / 64),
+                                 s->nslots, false,
                                  s->node64 ? s->node64_cap : 0);
     if (used_bytes) {
       uint64_t u = 0;

@@ -35,9 +35,9 @@
 // time (one 64-byte scalar load).
 //
 // After the walk (STEP): Hernquist halos, kick-drift, gas drag, pad density
-// and specific energy (nbody_bh.wgsl:144-180), then per-block FP64 partial
-// sums of kinetic, pair-potential and halo energy, momentum, the reference's
-// specific-energy sum and mass (fixed-order, deterministic).
+// and specific energy (nbody_bh.wgsl:144-180), then per-wavefront FP64
+// partial sums of kinetic, pair-potential and halo energy, momentum, the
+// reference's specific-energy sum and mass (fixed-order, deterministic).
 #include "gs_internal.h"
 
 namespace {
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   const uint32_t i = (blk - sl.wblock0) * blockDim.x + threadIdx.x;  // slot-local particle
   const bool valid = i < n;
   if ((blk - sl.wblock0) * blockDim.x >= n) {  // a block of empty slot capacity
-    if (STEP && threadIdx.x < GS_NRED) red[(size_t)blk * GS_NRED + threadIdx.x] = 0.0;
+    if (STEP && lane < GS_NRED) red[((size_t)blk * waves + w) * GS_NRED + lane] = 0.0;
     return;
   }
   const float4 p = valid ? posm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -596,17 +596,16 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     r_espec = (double)e;
     r_m = m;
   }
+  // one set of partial sums per wavefront (fixed order: butterfly within the
+  // wavefront, then the wavefronts in index order in reduce_final_kernel), so
+  // no wavefront waits for the others of its workgroup
   double acc[GS_NRED] = {r_ekin, r_epot, r_ehalo, r_px, r_py, r_pz, r_espec, r_m};
-  __shared__ double s_red[16][GS_NRED];
 #pragma unroll
   for (int k = 0; k < GS_NRED; k++) acc[k] = wave_sum(acc[k]);
-  if (lane == 0)
-    for (int k = 0; k < GS_NRED; k++) s_red[w][k] = acc[k];
-  __syncthreads();
-  if (threadIdx.x < GS_NRED) {
-    double s = 0;
-    for (int k = 0; k < waves; k++) s += s_red[k][threadIdx.x];
-    red[(size_t)blk * GS_NRED + threadIdx.x] = s;
+  if (lane == 0) {
+    double* dst = red + ((size_t)blk * waves + w) * GS_NRED;
+#pragma unroll
+    for (int k = 0; k < GS_NRED; k++) dst[k] = acc[k];
   }
 }
 
@@ -648,6 +647,9 @@ hipError_t gs_launch_walk(gs_sim* s, bool integrate, bool count) {
 
 hipError_t gs_launch_reduce_final(gs_sim* s, uint32_t slot, double* out_dev) {
   const HostSlot& h = s->hs[slot];
-  reduce_final_kernel<<<1, 256, 0, s->stream>>>(s->red_part, h.d.wblock0, h.d.nwblocks, out_dev);
+  // the slot's wavefronts: walk_wg / 64 per walk block
+  const uint32_t wpb = s->walk_wg / 64;
+  reduce_final_kernel<<<1, 256, 0, s->stream>>>(s->red_part, h.d.wblock0 * wpb, h.d.nwblocks * wpb,
+                                                out_dev);
   return hipGetLastError();
 }
