@@ -17,11 +17,12 @@
 //                 error margin, integer soft-double within the margin of a
 //                 cell boundary or outside the box), code =
 //                 part1by2(qx) | qy<<1 | qz<<2; writes the
-//                 sort's (key, slot-local index) input, padding keys
-//                 0xffffffff (sorted last) up to the end of the slot's last
-//                 active chunk, and the slot's 4 digit histograms.
+//                 sort's keys, padding keys 0xffffffff (sorted last) up to
+//                 the end of the slot's last active chunk, and the slot's 4
+//                 digit histograms.  (The sort's first pass pairs each key
+//                 with its slot-local index itself.)
 // All three are HBM-streaming: 16 B read per particle for bbox, 16 B read +
-// 8 B written for encode.
+// 4 B written for encode.
 #include "gs_internal.h"
 #include "quantize.h"
 #include "softdouble.h"
@@ -265,7 +266,6 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
                                                      const DevSlot* __restrict__ slots,
                                                      const uint2* __restrict__ chunk_tab,
                                                      uint32_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ vals,
                                                      uint32_t* __restrict__ raw,
                                                      DevScalars* scs) {
   const uint2 ct = chunk_tab[blockIdx.x];
@@ -293,7 +293,6 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
       if (KEEP_RAW) raw[sl.poff + i] = code;
     }
     keys[sl.poff + i] = code;
-    vals[sl.poff + i] = i;
     count_digits<BITS>(hist, code);
   }
   flush_hist<BITS>(hist, sc);
@@ -301,12 +300,11 @@ __global__ __launch_bounds__(256) void encode_kernel(const float4* __restrict__ 
 
 // digit histograms of caller-supplied keys in slot 0 (gs_radix_sort_indices
 // and its allocation-free form): keys[i] = src[i] below n, padding keys
-// 0xffffffff up to the end of the last active chunk (inside the capacity),
-// vals[i] = i.  src may equal keys.
+// 0xffffffff up to the end of the last active chunk (inside the capacity).
+// src may equal keys.
 template <int BITS>
 __global__ __launch_bounds__(256) void hist_kernel(const uint32_t* src,
                                                    uint32_t* keys,
-                                                   uint32_t* __restrict__ vals,
                                                    const DevSlot* __restrict__ slots,
                                                    DevScalars* scs) {
   const DevSlot sl = slots[0];
@@ -318,7 +316,6 @@ __global__ __launch_bounds__(256) void hist_kernel(const uint32_t* src,
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
     const uint32_t code = i < sl.n ? src[sl.poff + i] : 0xffffffffu;
     keys[sl.poff + i] = code;
-    vals[sl.poff + i] = i;
     count_digits<BITS>(hist, code);
   }
   flush_hist<BITS>(hist, scs);
@@ -342,10 +339,10 @@ template <int BITS>
 void launch_encode(gs_sim* s, const float4* posm, bool keep_raw) {
   if (keep_raw)
     encode_kernel<BITS, true><<<s->chunks, 256, 0, s->stream>>>(
-        posm, s->slots, s->chunk_tab, s->keys[0], s->vals[0], s->codes_raw, s->sc);
+        posm, s->slots, s->chunk_tab, s->keys[0], s->codes_raw, s->sc);
   else
     encode_kernel<BITS, false><<<s->chunks, 256, 0, s->stream>>>(
-        posm, s->slots, s->chunk_tab, s->keys[0], s->vals[0], s->codes_raw, s->sc);
+        posm, s->slots, s->chunk_tab, s->keys[0], s->codes_raw, s->sc);
 }
 
 }  // namespace
@@ -379,9 +376,9 @@ hipError_t gs_launch_encode(gs_sim* s, const float4* posm, bool keep_raw) {
 hipError_t gs_launch_hist_bufs(const SortBufs& b, const uint32_t* src, uint32_t cap) {
   const uint32_t g = grid_for(cap, 256, 4096);
   if (b.radix_bits == 10)
-    hist_kernel<10><<<g, 256, 0, b.stream>>>(src, b.keys[0], b.vals[0], b.slots, b.sc);
+    hist_kernel<10><<<g, 256, 0, b.stream>>>(src, b.keys[0], b.slots, b.sc);
   else
-    hist_kernel<8><<<g, 256, 0, b.stream>>>(src, b.keys[0], b.vals[0], b.slots, b.sc);
+    hist_kernel<8><<<g, 256, 0, b.stream>>>(src, b.keys[0], b.slots, b.sc);
   return hipGetLastError();
 }
 

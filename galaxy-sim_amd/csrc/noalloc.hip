@@ -16,11 +16,11 @@
 //   1. clear_kernel: zero the look-back words of the tiles in use, the
 //      tickets and the histograms; write the slot descriptor (n, capacity n)
 //      and the tile table;
-//   2. hist_kernel (encode.hip): keys[0] = codes, perm = identity, digit
-//      histograms;
+//   2. hist_kernel (encode.hip): keys[0] = codes, digit histograms;
 //   3. onesweep_kernel x 4 (sort.hip): keys[0] -> keys[1] -> keys[0] ...,
-//      values perm -> vals -> perm -> vals -> perm, so the last pass leaves
-//      the gather permutation in the caller's perm.
+//      values (the first pass pairs each key with its index) -> vals ->
+//      perm -> vals -> perm, so the last pass leaves the gather permutation
+//      in the caller's perm.
 // Every call on a device shares that device's scratch: calls that overlap on
 // one device (different streams, or threads) race, and must be serialised by
 // the caller.

@@ -25,8 +25,9 @@
 // (apply_permutation, nbody_bh.wgsl:319-326), so sorted positions land in the
 // other ping-pong buffer without a separate pass over the index array.
 //
-// HBM per pass: 8 B read + 8 B written per key (+ 32 B gathered + 32 B
-// written per particle in the last pass).  8-bit digits write 256 runs of
+// HBM per pass: 8 B read + 8 B written per key (4 B read in the first pass,
+// which pairs each key with its slot-local index itself; + 32 B gathered +
+// 32 B written per particle in the last pass).  8-bit digits write 256 runs of
 // ~16 keys per tile; 10-bit digits save a pass but write 1024 runs of ~4.
 #include "gs_internal.h"
 
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
       const uint32_t dd = (key[r] >> shift) & (RADIX - 1);
       const uint32_t pos = s_dstart[dd] + s_wh[w][dd] + rank[r];
       s_k[pos] = key[r];
-      s_v[pos] = vin[idx];
+      s_v[pos] = pass == 0 ? idx : vin[idx];  // the first pass pairs keys with their index
     }
   }
   __syncthreads();
