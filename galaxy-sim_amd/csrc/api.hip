@@ -326,7 +326,19 @@ hipGraphExec_t step_graph(gs_sim* s, bool sort, int part) {
   return ex;
 }
 
+// Marks the walks launched or captured in its scope as walking a tree built
+// from the current positions; every other walk sees walk_fresh false, also
+// when the scope is left by an error.
+struct FreshWalk {
+  gs_sim* s;
+  FreshWalk(gs_sim* sim, bool fresh) : s(sim) { s->walk_fresh = fresh; }
+  ~FreshWalk() { s->walk_fresh = false; }
+};
+
 void run_step(gs_sim* s, bool sort) {
+  // the walk of a rebuilding step sees the tree built from its positions
+  // (captured into the step graphs of that kind)
+  FreshWalk fresh(s, sort);
   if (use_graph(s) && (s->flags & GS_FLAG_TIMING)) {
     // two replays with events between them: the walk and the whole step are
     // timed on the stream that runs them
@@ -338,7 +350,7 @@ void run_step(gs_sim* s, bool sort) {
     }
     {
       Stage w(s, GS_T_WALK);
-      check(hipGraphLaunch(step_graph(s, false, 2), s->stream), "graph launch");
+      check(hipGraphLaunch(step_graph(s, sort, 2), s->stream), "graph launch");
     }
     s->cur ^= 1;
   } else if (use_graph(s)) {

@@ -349,6 +349,10 @@ struct gs_sim {
   uint2* chunk_tab = nullptr;       // chunks: (slot, local start)
   uint32_t* tile_tab = nullptr;     // sort_tiles: slot << 16 | local tile
   uint32_t* wblock_slot = nullptr;  // walk_blocks: slot of each walk block
+  // passed to the next walk launch (direct or captured): the tree was built
+  // on the device from the current positions of every slot (set only for the
+  // walk of a step that rebuilds)
+  bool walk_fresh = false;
   uint2* big_leaf = nullptr;        // (slot, node) of large leaves
   uint32_t big_cap = 0;             // ptot / GS_BIG_LEAF + 64 entries
 

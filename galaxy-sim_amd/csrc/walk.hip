@@ -219,7 +219,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // once per leaf.  The
 // kernel picks the instance once per block, so the node loop carries no
 // test of it.
-template <bool EXACT, bool COUNT, bool FASTLEAF, bool GENERIC>
+template <bool EXACT, bool COUNT, bool FASTLEAF, bool GENERIC, bool OWNOPEN>
 __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
                                               const float* __restrict__ nsize,
                                               const float4* __restrict__ posm, float4 p,
@@ -332,19 +332,24 @@ __device__ __forceinline__ void walk_particle(const NodeRec* __restrict__ nodes,
       const uint32_t count = gs_meta_count(meta);
       // the leaf holding a lane's own particle sets its density.  FASTLEAF: a
       // lane that opens its own leaf takes back the self potential that the
-      // member loop below adds.
-      const uint64_t own = act & __builtin_amdgcn_ballot_w64(i - first < count);
-      if (own != 0) {  // a scalar test first: most leaves hold no lane's particle
-        if (__builtin_amdgcn_inverse_ballot_w64(own)) {
-          const float sz = nsize[gs_node(nd)];
-          dens = (float)count / ((sz * sz) * sz);
-          if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
+      // member loop below adds.  OWNOPEN (no lane can accept its own leaf, see
+      // walk_kernel): only a leaf that some lane opens can be a lane's own.
+      auto own_leaf = [&]() {
+        const uint64_t own = act & __builtin_amdgcn_ballot_w64(i - first < count);
+        if (own != 0) {  // a scalar test first: most leaves hold no lane's particle
+          if (__builtin_amdgcn_inverse_ballot_w64(own)) {
+            const float sz = nsize[gs_node(nd)];
+            dens = (float)count / ((sz * sz) * sz);
+            if (FASTLEAF && __builtin_amdgcn_inverse_ballot_w64(need_mask)) pot = pot + selfpot;
+          }
         }
-      }
+      };
+      if (!OWNOPEN) own_leaf();
       uint32_t lcp = 0;
       if (need_mask == 0) {
         // no lane opens the leaf
       } else {
+      if (OWNOPEN) own_leaf();
       // FASTLEAF: the transposed form's sums come back in d, added below
       // where the forms meet (d = 0 after the broadcast loop, which adds into
       // the accumulators member by member): the accumulators keep one
@@ -449,7 +454,8 @@ __global__ __launch_bounds__(1024) void walk_kernel(
     uint32_t* __restrict__ inter_all, const NodeRec* __restrict__ pool,
     const float* __restrict__ nsize_all, const DevSlot* __restrict__ slots, const uint32_t* __restrict__ wblock_slot,
     const DevParams* __restrict__ prms, const DevScalars* __restrict__ scs,
-    const uint32_t* __restrict__ massless, DevBatch* batch, double* __restrict__ red) {
+    const uint32_t* __restrict__ massless, DevBatch* batch, double* __restrict__ red,
+    uint32_t fresh_tree) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
   // transposed-leaf staging of this wavefront (dynamic LDS sized to the
@@ -497,6 +503,28 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   // without both tests unless the slot needs either
   const bool generic = scs[slot].min_s2 < qeps2 || massless[slot] != 0u;
 
+  // OWNOPEN: no lane can accept the leaf holding its own particle as a
+  // monopole, so only a leaf that some lane opens can be a lane's own (the
+  // own-leaf density test then runs on opened leaves only).  When the tree
+  // was built from the current positions (fresh_tree), a particle lies in
+  // its own leaf's cell, at most sqrt(3) size from the leaf's COM, and
+  // accepting needs s^2 < theta^2 (3 s^2 + eps^2); no node of size^2 >=
+  // min_s2 passes that when 3 theta^2 < 1 and min_s2 (1 - 3 theta^2) >=
+  // theta^2 eps^2.  The margins (3.03, 1.01) cover the f32 roundings of d^2,
+  // of the COM and of the differences, the last bounded by requiring
+  // coordinates |x| <= M with M^2 <= 1e9 min_s2 (rounding of order M 2^-24
+  // against a cell of size >= sqrt(min_s2)).  NaN anywhere fails a compare.
+  bool ownopen = false;
+  if (!EXACT && !COUNT && fresh_tree && !generic) {
+    const DevScalars* sc = scs + slot;
+    const float th2 = prm->theta2, t3 = 3.03f * th2, rs = sc->level_size[0];
+    float m = 0.f;
+    for (int a = 0; a < 3; a++)
+      m = fmaxf(m, fmaxf(fabsf(sc->min_f32[a]), fabsf(sc->min_f32[a] + rs)));
+    ownopen = t3 < 1.0f && sc->min_s2 * (1.0f - t3) >= 1.01f * th2 * eps2 &&
+              m * m <= 1e9f * sc->min_s2;
+  }
+
   // FASTLEAF adds a lane's own j == i pair as r * f with r = 0, an exact zero
   // only while f = m rsq(eps2)^3 (and m rsq(eps2)) is finite; a huge mass or
   // a tiny eps makes it Inf and the zero a NaN, where the reference skips the
@@ -511,15 +539,17 @@ __global__ __launch_bounds__(1024) void walk_kernel(
 
   float ax = 0.f, ay = 0.f, az = 0.f, pot = 0.f, dens = 0.f;
   uint32_t cnt = 0;
-#define GS_WALK(FL, QC)                                                                      \
-  walk_particle<EXACT, COUNT, FL, QC>(nodes, nsize, posm, p, i, valid, G, eps2, qsize, ax,   \
-                                      ay, az, pot, dens, cnt, wl)
+#define GS_WALK(FL, QC, OO)                                                                  \
+  walk_particle<EXACT, COUNT, FL, QC, OO>(nodes, nsize, posm, p, i, valid, G, eps2, qsize,   \
+                                          ax, ay, az, pot, dens, cnt, wl)
   if (fastleaf) {
-    if (generic) GS_WALK(true, true);
-    else GS_WALK(true, false);
+    if (generic) GS_WALK(true, true, false);
+    else if (ownopen) GS_WALK(true, false, true);
+    else GS_WALK(true, false, false);
   } else {
-    if (generic) GS_WALK(false, true);
-    else GS_WALK(false, false);
+    if (generic) GS_WALK(false, true, false);
+    else if (ownopen) GS_WALK(false, false, true);
+    else GS_WALK(false, false, false);
   }
 #undef GS_WALK
   if (!EXACT) {
@@ -632,7 +662,7 @@ hipError_t launch(gs_sim* s) {
   walk_kernel<EXACT, STEP, COUNT><<<s->walk_blocks, s->walk_wg, lds, s->stream>>>(
       s->posm[c], s->velp[c], s->posm[c ^ 1], s->velp[c ^ 1], s->energy, s->acc, s->dens,
       s->inter, s->nodes, s->nsize, s->slots, s->wblock_slot, s->prm, s->sc, s->massless,
-      s->batch, s->red_part);
+      s->batch, s->red_part, s->walk_fresh ? 1u : 0u);
   return hipGetLastError();
 }
 

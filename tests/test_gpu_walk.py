@@ -344,3 +344,35 @@ def test_walk_step_is_the_reference_step():
         b.destroy()
     want, _, _, _, _ = oracle.walk(before.reshape(-1), nodes, op)
     assert np.array_equal(after.reshape(-1).view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("theta,eps", [(0.5, 0.005), (0.57, 0.001), (0.7, 0.01)])
+def test_fast_step_own_leaf_density(theta, eps):
+    """Default path, one rebuilding step: the density of each particle's own
+    leaf (seen in the gas pad, -(1 + density)) equals the oracle's bit for
+    bit.  At theta^2 < 1/3 the walk tests the own leaf only on leaves some
+    lane opens (no particle can accept its own leaf); 0.57 sits just inside
+    that bound, 0.7 outside it."""
+    n = 30000
+    banner(f"fast step own-leaf density theta={theta} eps={eps}")
+    data = ic.make("disk_500k", n=n)
+    gas = data[:, 7] < -0.5
+    assert gas.sum() > 5000
+    base, halo, op = _params(n, theta=theta, epsilon=eps)
+    sim = galaxy_hip.NBodyGPU(data, bucket=16)
+    try:
+        sim.update_params(haloCenters=halo, **base)
+        sim.step(1, sort_every=1)
+        parts = sim.particles()
+    finally:
+        sim.destroy()
+    ref = oracle.sort_and_build(data.reshape(-1), n, bucket=16)
+    pout, _, _, _, _ = oracle.walk(ref["sorted"], ref["nodes"], op)
+    want = pout.reshape(-1, 8)
+    # same particle order (the step's sort of identical positions)
+    assert np.array_equal(parts[:, 3].view(np.uint32), want[:, 3].view(np.uint32))
+    g = want[:, 7] < -0.5
+    assert g.sum() == gas.sum()
+    assert np.array_equal(parts[g, 7].view(np.uint32), want[g, 7].view(np.uint32))
+    if 3 * theta * theta < 1:  # then every particle visits its own leaf
+        assert (want[g, 7] < -1.0).all()
