@@ -25,6 +25,12 @@ namespace {
 
 constexpr int LEAF_BATCH = 8;
 
+// an empty asm that takes a loaded value in and out: the load issues before
+// this point, and the value's uses cannot pull it past it
+__device__ __forceinline__ void pin_value(float4& v) {
+  asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+}
+
 // a node stored with zero mass marks its slot, so that the walk runs its
 // variant with the massless-node test (nbody_bh.wgsl:86)
 __device__ __forceinline__ void store_com(NodeRec* nodes, uint32_t idx, double m, double cx,
@@ -40,12 +46,17 @@ __device__ __forceinline__ void sum_members(const float4* __restrict__ pp, uint3
                                             uint32_t last, double& m, double& cx, double& cy,
                                             double& cz) {
   // members are loaded LEAF_BATCH at a time ahead of the (sequential,
-  // order-fixed) FP64 sums, so the loads of a batch are in flight together
+  // order-fixed) FP64 sums, so the loads of a batch are in flight together:
+  // every load is unconditional (past the leaf it re-reads the last member,
+  // whose value is not used) and the values pass through empty asm, so the
+  // compiler can neither make a load conditional nor sink it to its use
+  // (either would wait for the loads one by one)
   for (uint32_t j0 = j; j0 < last; j0 += LEAF_BATCH) {
     float4 q[LEAF_BATCH];
 #pragma unroll
-    for (int k = 0; k < LEAF_BATCH; k++)
-      q[k] = j0 + k < last ? pp[j0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < LEAF_BATCH; k++) q[k] = pp[min(j0 + (uint32_t)k, last - 1u)];
+#pragma unroll
+    for (int k = 0; k < LEAF_BATCH; k++) pin_value(q[k]);
 #pragma unroll
     for (int k = 0; k < LEAF_BATCH; k++) {
       if (j0 + k >= last) break;
@@ -122,10 +133,17 @@ __global__ __launch_bounds__(256) void refresh_big_kernel(const float4* __restri
     const float4* pp = posm + sl.poff;
     double acc = 0.0;  // lane k < 4: sum k of (m, cx, cy, cz)
     for (uint32_t c = first; c < last; c += CH) {
+      // the round's loads first (unconditional, pinned as in sum_members;
+      // past the leaf they re-read its last member and weigh 0)
+      float4 qv[CH / 64];
+#pragma unroll
+      for (uint32_t k = 0; k < CH / 64; k++) qv[k] = pp[min(c + k * 64 + (uint32_t)lane, last - 1u)];
+#pragma unroll
+      for (uint32_t k = 0; k < CH / 64; k++) pin_value(qv[k]);
 #pragma unroll
       for (uint32_t k = 0; k < CH / 64; k++) {
         const uint32_t t = k * 64 + (uint32_t)lane;
-        const float4 q = c + t < last ? pp[c + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 q = c + t < last ? qv[k] : make_float4(0.f, 0.f, 0.f, 0.f);
         const double pm = (double)q.w;
         s_val[w][t][0] = pm;
         s_val[w][t][1] = __dmul_rn((double)q.x, pm);
@@ -165,14 +183,23 @@ __device__ __forceinline__ void refresh_node(NodeRec* __restrict__ nodes, uint32
   if (gs_meta_is_leaf((uint32_t)inf.z)) return;
   double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
   const uint32_t nc = gs_meta_nc((uint32_t)inf.z);
-  const uint32_t fc = gs_node(inf.y) + 1u - nc;  // y = last child
-  for (uint32_t c = fc; c < fc + nc; c++) {
-    const float4 q = nodes[c].cm;
-    const double qm = (double)q.w;
-    m = __dadd_rn(m, qm);
-    cx = __dadd_rn(cx, __dmul_rn((double)q.x, qm));
-    cy = __dadd_rn(cy, __dmul_rn((double)q.y, qm));
-    cz = __dadd_rn(cz, __dmul_rn((double)q.z, qm));
+  if (nc > 0) {
+    const uint32_t fc = gs_node(inf.y) + 1u - nc;  // y = last child
+    // the (at most 8) children are loaded together, as in sum_members
+    float4 q[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) q[k] = nodes[fc + min(k, nc - 1u)].cm;
+#pragma unroll
+    for (int k = 0; k < 8; k++) pin_value(q[k]);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+      if (k >= nc) break;
+      const double qm = (double)q[k].w;
+      m = __dadd_rn(m, qm);
+      cx = __dadd_rn(cx, __dmul_rn((double)q[k].x, qm));
+      cy = __dadd_rn(cy, __dmul_rn((double)q[k].y, qm));
+      cz = __dadd_rn(cz, __dmul_rn((double)q[k].z, qm));
+    }
   }
   store_com(nodes, idx, m, cx, cy, cz, massless);
 }

@@ -101,13 +101,24 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
   const uint32_t base = tile * GS_SORT_TILE;
   const uint64_t below_mask = (1ull << lane) - 1;
 
-  uint32_t key[ITEMS], rank[ITEMS];
+  // every key and value load of the tile issues before the first is used:
+  // one memory latency per tile (ranking item r before loading item r + 1
+  // would wait for each load in turn, as the loads are conditional and
+  // cannot be moved above the previous item's ballots), and the values,
+  // needed only after the look-back, add no latency of their own
+  uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; r++) {
     const uint32_t idx = base + (uint32_t)w * (64 * ITEMS) + (uint32_t)r * 64 + lane;
     const bool valid = idx < n;
-    const uint32_t k = valid ? kin[idx] : 0u;
-    key[r] = k;
+    key[r] = valid ? kin[idx] : 0u;
+    val[r] = pass != 0 && valid ? vin[idx] : idx;  // the first pass pairs keys with their index
+  }
+#pragma unroll
+  for (int r = 0; r < ITEMS; r++) {
+    const uint32_t idx = base + (uint32_t)w * (64 * ITEMS) + (uint32_t)r * 64 + lane;
+    const bool valid = idx < n;
+    const uint32_t k = key[r];
     const uint32_t d = (k >> shift) & (RADIX - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -202,25 +213,84 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
       const uint32_t dd = (key[r] >> shift) & (RADIX - 1);
       const uint32_t pos = s_dstart[dd] + s_wh[w][dd] + rank[r];
       s_k[pos] = key[r];
-      s_v[pos] = pass == 0 ? idx : vin[idx];  // the first pass pairs keys with their index
+      s_v[pos] = val[r];
     }
   }
   __syncthreads();
 
   const uint32_t tile_n = n - base < (uint32_t)GS_SORT_TILE ? n - base : (uint32_t)GS_SORT_TILE;
-  for (uint32_t i = tid; i < tile_n; i += GS_SORT_BLOCK) {
-    const uint32_t k = s_k[i];
-    const uint32_t dd = (k >> shift) & (RADIX - 1);
-    const uint32_t out = s_gbase[dd] + (i - s_dstart[dd]);
-    const uint32_t v = s_v[i];
-    if (out < n) {
-      kout[out] = k;
-      vout[out] = v;
-      if (GATHER && v < n) {
-        pout[sl.poff + out] = pin[sl.poff + v];
-        qout[sl.poff + out] = qin[sl.poff + v];
-        if (sin) sout[sl.poff + out] = sin[sl.poff + v];  // stellar records, when present
+  if (!GATHER) {
+    for (uint32_t i = tid; i < tile_n; i += GS_SORT_BLOCK) {
+      const uint32_t k = s_k[i];
+      const uint32_t dd = (k >> shift) & (RADIX - 1);
+      const uint32_t out = s_gbase[dd] + (i - s_dstart[dd]);
+      if (out < n) {
+        kout[out] = k;
+        vout[out] = s_v[i];
       }
+    }
+    return;
+  }
+  // the gather: the records of GB items are loaded before the first is
+  // stored (one memory latency per GB items, not one per item).  Every
+  // item loads (one without a record to move, past the tile or the
+  // particles, loads record 0 of the slot, which exists in an active tile)
+  // so the loads run unconditionally and their values stay in registers.
+  constexpr int GB = 8;
+  for (uint32_t i0 = tid; i0 < tile_n; i0 += GS_SORT_BLOCK * GB) {
+    uint32_t src[GB], dst[GB];
+    bool move[GB];
+#pragma unroll
+    for (int b = 0; b < GB; b++) {
+      const uint32_t i = i0 + (uint32_t)b * GS_SORT_BLOCK;
+      src[b] = 0;
+      dst[b] = 0;
+      move[b] = false;
+      if (i < tile_n) {
+        const uint32_t k = s_k[i];
+        const uint32_t dd = (k >> shift) & (RADIX - 1);
+        const uint32_t out = s_gbase[dd] + (i - s_dstart[dd]);
+        const uint32_t v = s_v[i];
+        if (out < n) {
+          kout[out] = k;
+          vout[out] = v;
+          if (v < n) {
+            src[b] = v;
+            dst[b] = out;
+            move[b] = true;
+          }
+        }
+      }
+    }
+    float4 pr[GB], qr[GB];
+#pragma unroll
+    for (int b = 0; b < GB; b++) {
+      pr[b] = pin[sl.poff + src[b]];
+      qr[b] = qin[sl.poff + src[b]];
+    }
+    // the values pass through empty asm here, so the compiler cannot sink
+    // each load into the conditional store that uses it (which would wait
+    // for each load in turn)
+#pragma unroll
+    for (int b = 0; b < GB; b++)
+      asm volatile("" : "+v"(pr[b].x), "+v"(pr[b].y), "+v"(pr[b].z), "+v"(pr[b].w),
+                   "+v"(qr[b].x), "+v"(qr[b].y), "+v"(qr[b].z), "+v"(qr[b].w));
+#pragma unroll
+    for (int b = 0; b < GB; b++)
+      if (move[b]) {
+        pout[sl.poff + dst[b]] = pr[b];
+        qout[sl.poff + dst[b]] = qr[b];
+      }
+    if (sin) {  // stellar records, when present
+      float4 sr[GB];
+#pragma unroll
+      for (int b = 0; b < GB; b++) sr[b] = sin[sl.poff + src[b]];
+#pragma unroll
+      for (int b = 0; b < GB; b++)
+        asm volatile("" : "+v"(sr[b].x), "+v"(sr[b].y), "+v"(sr[b].z), "+v"(sr[b].w));
+#pragma unroll
+      for (int b = 0; b < GB; b++)
+        if (move[b]) sout[sl.poff + dst[b]] = sr[b];
     }
   }
 }
