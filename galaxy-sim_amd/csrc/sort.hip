@@ -120,12 +120,16 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
     const bool valid = idx < n;
     const uint32_t k = key[r];
     const uint32_t d = (k >> shift) & (RADIX - 1);
+    // the lanes whose digit equals this lane's: per bit, the lanes that
+    // agree with this lane's bit, ~(ballot(bit) ^ (bit ? ~0 : 0)), the
+    // 0 / -1 word of a signed one-bit extract in both halves (a select per
+    // half, as `bit ? bb : ~bb` compiles, costs twice the instructions)
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < BITS; b++) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
+      const uint32_t sb = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, (uint32_t)b, 1u);
+      const uint64_t bb = __ballot(sb != 0u);
+      peers &= ~(bb ^ (((uint64_t)sb << 32) | sb));
     }
     const uint32_t below = (uint32_t)__popcll(peers & below_mask);
     uint32_t prior = 0;
