@@ -167,7 +167,9 @@ __global__ __launch_bounds__(GS_SORT_BLOCK) void onesweep_kernel(
   for (int j = 0; j < DPT; j++) {
     const uint32_t d = (uint32_t)tid * DPT + j;
     uint32_t excl = 0;
-    if (tile > 0) {
+    // a digit that no key of the slot holds (most top digits of the 30-bit
+    // codes) needs no prefix: no tile reads its status or its offset
+    if (tile > 0 && hist[j] != 0) {
       // a slot's tiles are consecutive; LB predecessors are read per round
       // (independent loads in flight together) and consumed nearest first
       // up to the first inclusive prefix or the first unpublished tile
