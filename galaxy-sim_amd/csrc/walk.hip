@@ -471,13 +471,19 @@ __global__ __launch_bounds__(1024) void walk_kernel(
   // each XCD has its own L2), while the runs still interleave over the whole
   // particle range, so every XCD walks a mix of dense and sparse regions.
   // The remap is a permutation of the grid (a partial last group keeps the
-  // identity order).
+  // identity order).  The work is then taken from the top of the particle
+  // range down (the first workgroup walks the last block): measured faster
+  // than ascending order on every workload larger than the launch latency
+  // (N = 500,000 disk 3.7 %, 64-galaxy pool 2.0 %, refresh-only steps of
+  // the 4,194,304 collision 2.5 %, its rebuilding steps 0.6 %), with the same
+  // results (each block's walk is independent of the order).
   constexpr uint32_t XCD_RUN = 64;
   uint32_t blk = blockIdx.x;
   if (blk < gridDim.x / (8 * XCD_RUN) * (8 * XCD_RUN)) {
     const uint32_t r = blk % (8 * XCD_RUN);
     blk = blk - r + (r % 8) * XCD_RUN + r / 8;
   }
+  blk = gridDim.x - 1u - blk;
   const uint32_t slot = wblock_slot[blk];
   const DevSlot sl = slots[slot];
   const DevParams* __restrict__ prm = prms + slot;
